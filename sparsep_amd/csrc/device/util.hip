@@ -1,0 +1,104 @@
+// Utility kernels: zero fill, cache flush, bandwidth-probe copy and the
+// partial-y reduction of the 2D kernels.
+#include "common.hpp"
+#include "plan.hpp"
+
+namespace spd {
+
+__global__ void fill_zero_kernel(uint4 *__restrict__ p, uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ void fill_zero_bytes_kernel(uint8_t *__restrict__ p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Overwrites the buffer so that earlier lines are evicted from L2 and MALL.
+__global__ void flush_kernel(uint4 *__restrict__ p, uint64_t n16, uint32_t salt) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        p[i] = make_uint4((uint32_t)i ^ salt, salt, (uint32_t)(i >> 32), ~salt);
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void copy_kernel(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = __builtin_nontemporal_load(src + i);
+}
+
+// y[i] = sum over v of ypart[v*stride + i] in part order (the host merge of
+// spmv/2D/DCSR/host/app.c:340-351 and its RBD/BD twins).
+template <typename T>
+__global__ void reduce_parts_kernel(const acc_t<T> *__restrict__ ypart, uint32_t vert, uint64_t stride,
+                                    T *__restrict__ y, uint64_t n, int k, int ldy) {
+    using A = acc_t<T>;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * (uint64_t)k) return;
+    const uint64_t row = i / k, q = i % k;
+    A s = ypart[i];
+    for (uint32_t v = 1; v < vert; v++) s = s + ypart[(uint64_t)v * stride + i];
+    y[row * ldy + q] = to_val<T>(s);
+}
+
+}  // namespace spd
+
+using namespace spd;
+
+static unsigned grid_for(uint64_t n, unsigned block) {
+    uint64_t g = (n + block - 1) / block;
+    if (g > 65536) g = 65536;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+int launch_fill_zero(hipStream_t s, void *ptr, uint64_t bytes) {
+    if (bytes == 0) return SP_OK;
+    const uint64_t n16 = bytes / 16;
+    if (((uintptr_t)ptr & 15) == 0 && n16) {
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(n16, 256)), dim3(256), 0, s, (uint4 *)ptr, n16);
+        const uint64_t rem = bytes - n16 * 16;
+        if (rem)
+            hipLaunchKernelGGL(fill_zero_bytes_kernel, dim3(1), dim3(256), 0, s, (uint8_t *)ptr + n16 * 16, rem);
+    } else {
+        hipLaunchKernelGGL(fill_zero_bytes_kernel, dim3((unsigned)((bytes + 255) / 256)), dim3(256), 0, s,
+                           (uint8_t *)ptr, bytes);
+    }
+    return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+}
+
+int launch_flush(hipStream_t s, void *buf, uint64_t bytes, uint32_t salt) {
+    const uint64_t n16 = bytes / 16;
+    if (!n16) return SP_OK;
+    hipLaunchKernelGGL(flush_kernel, dim3(grid_for(n16, 256)), dim3(256), 0, s, (uint4 *)buf, n16, salt);
+    return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+}
+
+int launch_copy(hipStream_t s, const void *src, void *dst, uint64_t bytes) {
+    const uint64_t n16 = bytes / 16;
+    if (!n16) return SP_OK;
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n16, 256)), dim3(256), 0, s, (const u32x4 *)src,
+                       (u32x4 *)dst, n16);
+    return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+}
+
+int launch_reduce_parts(hipStream_t s, int dtype, const void *ypart, uint32_t vert, uint64_t stride,
+                        void *y, uint64_t n, int k, int ldy) {
+    const uint64_t total = n * (uint64_t)k;
+    if (!total) return SP_OK;
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    switch (dtype) {
+    case SP_I8: hipLaunchKernelGGL(reduce_parts_kernel<int8_t>, grid, block, 0, s, (const uint32_t *)ypart, vert, stride, (int8_t *)y, n, k, ldy); break;
+    case SP_I16: hipLaunchKernelGGL(reduce_parts_kernel<int16_t>, grid, block, 0, s, (const uint32_t *)ypart, vert, stride, (int16_t *)y, n, k, ldy); break;
+    case SP_I32: hipLaunchKernelGGL(reduce_parts_kernel<int32_t>, grid, block, 0, s, (const uint32_t *)ypart, vert, stride, (int32_t *)y, n, k, ldy); break;
+    case SP_I64: hipLaunchKernelGGL(reduce_parts_kernel<int64_t>, grid, block, 0, s, (const uint64_t *)ypart, vert, stride, (int64_t *)y, n, k, ldy); break;
+    case SP_F32: hipLaunchKernelGGL(reduce_parts_kernel<float>, grid, block, 0, s, (const float *)ypart, vert, stride, (float *)y, n, k, ldy); break;
+    case SP_F64: hipLaunchKernelGGL(reduce_parts_kernel<double>, grid, block, 0, s, (const double *)ypart, vert, stride, (double *)y, n, k, ldy); break;
+    default: return SP_ERR_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+}

@@ -1,0 +1,40 @@
+"""The C-ABI library loads and exports every function include/sparsep.h declares."""
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "sparsep.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set(re.findall(r"\b(sp_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
+
+
+def test_library_exports_all_declared_symbols(sp):
+    L = sp.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    assert len(declared_functions()) > 50
+
+
+def test_strerror_and_sizes(sp):
+    L = sp.lib()
+    assert L.sp_strerror(0) == b"ok"
+    assert [L.sp_dtype_size(d) for d in range(6)] == [1, 2, 4, 8, 4, 8]
+    assert L.sp_dtype_size(9) == 0
+
+
+def test_kernel_names(sp):
+    assert [sp.lib().sp_kernel_name(i).decode() for i in range(25)] == sp.KERNELS
+
+
+def test_ctx_without_gpu_reports_error(sp):
+    import torch
+    if torch.cuda.is_available():
+        return
+    import ctypes
+    p = ctypes.c_void_p()
+    rc = sp.lib().sp_ctx_create(0, ctypes.byref(p))
+    assert rc in (-6, -4)

@@ -1,0 +1,106 @@
+"""GPU parity of the CSR variants (CSR-row, CSR-nnz) against the oracle.
+
+Integer dtypes must be bit-exact with the reference spmv_host
+(spmv/1D/CSR-row/host/app.c:66-77); float dtypes must lie within the
+gamma(len+2)*sum|a_ij x_j| rounding bound of a long-double reference.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import assert_parity, assert_parity_multi, run
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [0, 1, 2, 3, 4, 5]
+ALGOS = [("vec1", dict(algo=1, lanes_per_row=1)), ("vec4", dict(algo=1, lanes_per_row=4)),
+         ("vec16", dict(algo=1, lanes_per_row=16)), ("vec64", dict(algo=1, lanes_per_row=64)),
+         ("seg4", dict(algo=2, items_per_lane=4)), ("seg8", dict(algo=2, items_per_lane=8)),
+         ("seg16", dict(algo=2, items_per_lane=16))]
+
+
+def test_stencil5_int32_bit_exact(sp, ctx):
+    """The 256x256 5-point stencil in INT32, every CSR algorithm."""
+    A = sp.gen_stencil5(256, 256, sp.I32, seed=1)
+    assert A.nnz == 326656
+    x = sp.gen_vector(sp.I32, A.ncols, sp.dist(0, -4, 4), seed=11)
+    for kern in ("CSR-row", "CSR-nnz"):
+        for name, kw in ALGOS:
+            y, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+            assert_parity(2, y, A, x, f"{kern}/{name}")
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+def test_rmat_all_dtypes(sp, ctx, dt):
+    A = sp.gen_rmat(12, 60000, dtype=dt, seed=5)
+    x = sp.gen_vector(dt, A.ncols, seed=6)
+    for kern in ("CSR-row", "CSR-nnz"):
+        for name, kw in ALGOS:
+            y, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+            assert_parity(dt, y, A, x, f"{kern}/{name}/dt{dt}")
+
+
+@pytest.mark.parametrize("dt", [0, 2, 4, 5])
+def test_uniform_and_banded(sp, ctx, dt):
+    for A in (sp.gen_uniform(5000, 7000, 0, 40, dtype=dt, seed=8), sp.gen_banded(6000, 32, 2, 64, dtype=dt, seed=9)):
+        x = sp.gen_vector(dt, A.ncols, seed=10)
+        for name, kw in ALGOS:
+            y, _ = run(sp, ctx, "CSR-nnz", A, x, sp.opts(**kw))
+            assert_parity(dt, y, A, x, name)
+
+
+def _coo(sp, nrows, ncols, rows, cols, dt, seed=3):
+    rng = np.random.default_rng(seed)
+    vals = rng.integers(-4, 5, size=len(rows)).astype(sp.NP_DTYPE[dt])
+    return sp.Coo.from_arrays(nrows, ncols, np.asarray(rows), np.asarray(cols), vals, sort=True)
+
+
+@pytest.mark.parametrize("dt", [1, 2, 4])
+def test_edge_cases(sp, ctx, dt):
+    cases = []
+    # no entries at all
+    cases.append(_coo(sp, 100, 50, [], [], dt))
+    # one long row spanning many tiles, surrounded by empty rows
+    n = 9000
+    cases.append(_coo(sp, 10, n, [5] * n, list(range(n)), dt))
+    # long first and last row, empty rows between, trailing empty rows
+    r = [0] * 5000 + [7] * 3 + [2999] * 6000
+    c = list(range(5000)) + [1, 2, 3] + list(range(6000))
+    cases.append(_coo(sp, 3500, 6000, r, c, dt))
+    # single row / single column
+    cases.append(_coo(sp, 1, 1, [0], [0], dt))
+    cases.append(_coo(sp, 3000, 1, list(range(0, 3000, 2)), [0] * 1500, dt))
+    for i, A in enumerate(cases):
+        x = np.random.default_rng(i).integers(-4, 5, size=A.ncols).astype(A.values.dtype)
+        for kern in ("CSR-row", "CSR-nnz"):
+            for name, kw in ALGOS:
+                y, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+                assert_parity(dt, y, A, x, f"case{i}/{kern}/{name}")
+
+
+@pytest.mark.parametrize("dt", [2, 4, 5])
+@pytest.mark.parametrize("k", [2, 3, 4, 8, 11])
+def test_spmm(sp, ctx, dt, k):
+    A = sp.gen_rmat(11, 20000, dtype=dt, seed=12)
+    X = np.stack([sp.gen_vector(dt, A.ncols, seed=20 + q) for q in range(k)], axis=1)
+    for name, kw in [("vec4", dict(algo=1, lanes_per_row=4)), ("seg4", dict(algo=2, items_per_lane=4)),
+                     ("seg8", dict(algo=2, items_per_lane=8))]:
+        Y, _ = run(sp, ctx, "CSR-nnz", A, X, sp.opts(**kw), k=k)
+        assert_parity_multi(dt, Y, A, X, name)
+
+
+def test_repeatable(sp, ctx):
+    """Deterministic: two runs give identical bits (float too)."""
+    A = sp.gen_rmat(14, 200000, dtype=sp.F32, seed=13)
+    x = sp.gen_vector(sp.F32, A.ncols, seed=14)
+    y1, plan = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=2))
+    y2, _ = run(sp, ctx, "CSR-nnz", A, x, plan=plan)
+    assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
+
+
+@pytest.mark.parametrize("dt", [2, 4])
+def test_rmat_full_size(sp, ctx, dt):
+    """R-MAT 2^20 x 2^20, 2^24 unique entries (the bench workload)."""
+    A = sp.gen_rmat(20, 1 << 24, dtype=dt, seed=2)
+    x = sp.gen_vector(dt, A.ncols, seed=22)
+    y, plan = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=2, items_per_lane=8))
+    assert_parity(dt, y, A, x, "rmat-full")
