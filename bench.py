@@ -52,13 +52,11 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import sparsep_amd.api as sp
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
     ctx = sp.Context(0)
     info = ctx.info()
     log(f"[bench] device {info['arch']} CUs={info['num_cus']} HBM={info['hbm_bytes'] / 2**30:.0f} GiB")
@@ -72,18 +70,18 @@ def main():
     plan = sp.Plan(ctx, "CSR-nnz", A, sp.opts(algo=sp.ALGO_SEGMENTED, items_per_lane=8))
     pinfo = plan.info()
     algo_bytes = pinfo["algo_bytes"]
-    xd = torch.from_numpy(x).cuda()
-    yd = torch.zeros(A.nrows, dtype=torch.float32, device="cuda")
+    xd = sp.DeviceArray.from_numpy(ctx, x)
+    yd = sp.DeviceArray(ctx, A.nrows, np.float32)
     flush = args.flush_mb << 20
-    torch.cuda.synchronize()
+    ctx.sync()
 
     # warmup (untimed)
     if args.warmup > 0:
-        plan.time_spmv(xd.data_ptr(), yd.data_ptr(), args.warmup, flush)
-    torch.cuda.synchronize()
+        plan.time_spmv(xd.addr, yd.addr, args.warmup, flush)
+    ctx.sync()
     t1 = time.perf_counter()
-    ms, ms_dom = plan.time_spmv_split(xd.data_ptr(), yd.data_ptr(), args.steps, flush)
-    torch.cuda.synchronize()
+    ms, ms_dom = plan.time_spmv_split(xd.addr, yd.addr, args.steps, flush)
+    ctx.sync()
     wall = time.perf_counter() - t1
     mean_ms = statistics.mean(ms)
     mean_dom = statistics.mean(ms_dom)
@@ -93,7 +91,7 @@ def main():
         f"dominant kernel {mean_dom * 1e3:.2f} us, {value:.0f} GB/s, wall {wall:.2f}s")
 
     # warm-cache (MALL-resident) figure, reported separately
-    ms_warm = plan.time_spmv(xd.data_ptr(), yd.data_ptr(), 20, 0)
+    ms_warm = plan.time_spmv(xd.addr, yd.addr, 20, 0)
     warm_gbs = algo_bytes / (statistics.median(ms_warm) * 1e-3) / 1e9
 
     # stream-copy probe: 1 GiB read + 1 GiB write
@@ -103,7 +101,7 @@ def main():
     # parity of the bench output against the oracle (float tolerance)
     import oracle_lib as orc
 
-    y = yd.cpu().numpy()
+    y = yd.numpy()
     C = A.to_csr()
     yref, absb = orc.ref_csr(dt, C.rowptr, C.colind, C.values, x)
     tol = orc.fp_tolerance(dt, np.diff(C.rowptr.astype(np.int64)), absb)

@@ -392,6 +392,56 @@ class Context:
         return list(ms)
 
 
+def gpu_present():
+    """True when a GPU kernel driver node exists (does not initialise HIP)."""
+    import os
+    return os.path.exists("/dev/kfd")
+
+
+class DeviceArray:
+    """Device buffer owned through the C ABI (sp_dmalloc / sp_dfree) with
+    synchronous host copies on the context stream."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctypes.c_void_p()
+        check(lib().sp_dmalloc(ctx.ptr, max(self.nbytes, 1), ctypes.byref(self.ptr)), "sp_dmalloc")
+
+    @staticmethod
+    def from_numpy(ctx, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(ctx, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        if self.nbytes:
+            check(lib().sp_memcpy_h2d(self.ctx.ptr, self.ptr, a.ctypes.data_as(ctypes.c_void_p), self.nbytes))
+
+    def fill_bytes(self, byte):
+        self.upload(np.full(self.nbytes, byte, np.uint8).view(self.dtype).reshape(self.shape))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().sp_memcpy_d2h(self.ctx.ptr, out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes))
+        return out
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib().sp_dfree(self.ctx.ptr, self.ptr)
+            self.ptr = None
+
+
 class Plan:
     """A device plan for one kernel variant (sp_plan_create)."""
 

@@ -1,0 +1,6 @@
+// Kernel instantiations for one value type (split per type so the build
+// compiles them in parallel).
+#include "launch.hpp"
+
+template int run_work_t<int8_t>(const Work &, sp_ctx *, const void *, void *, const uintptr_t *, const uintptr_t *,
+                            int, int, int);

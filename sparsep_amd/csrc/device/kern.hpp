@@ -1,0 +1,667 @@
+// gfx950 SpMV / SpMM kernels over a list of sub-problems.
+//
+// Every kernel reads a per-workgroup problem id (unit_pid) and a problem
+// descriptor (Prob), so one launch serves a 1D plan (one problem), a 2D
+// tiling (one problem per tile, writing partial y) or a batch of
+// independent SpMVs.  Work coordinates are stored per problem with one
+// extra closing entry: unit u of problem p uses coord[u + p] .. coord[u + p + 1].
+//
+// Device replacements of SparseP's DPU tasklet kernels:
+//   csr_merge   entry-balanced CSR (merge path over row ends + entries)
+//   csr_vector  row-granular CSR, L lanes per row
+//                 (spmv/1D/CSR-row/dpu/task.c:105-202)
+//   coo_seg     entry-balanced COO with a segmented reduction; rows that
+//               straddle tiles are carried to a fix-up pass
+//                 (spmv/1D/COO-nnz/dpu/task.c:21-679, COO-row/dpu/task.c:31-492)
+//   bcsr_vector register-blocked BCSR, L lanes per block row
+//                 (spmv/1D/BCSR-nnz/dpu/task.c:231-380)
+//   bcoo_seg    block-balanced BCOO with per-row segmented carries
+//                 (spmv/1D/BCOO-nnz/dpu/task.c:22-894)
+//   fixup       adds cross-tile carries in tile order and zeroes rows
+//               without entries (replaces the host merges
+//               spmv/1D/COO-nnz/host/app.c:275-289 and
+//               spmv/1D/BCOO-block/host/app.c:296-313)
+// All reductions have a fixed order, so results are run-to-run identical.
+#pragma once
+
+#include "common.hpp"
+
+namespace spd {
+
+constexpr int TPB = 256;
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+
+struct Prob {
+    const uint32_t *ptr;   // CSR rowptr / BCSR browptr
+    const uint32_t *rind;  // COO rowind / BCOO browind
+    const uint32_t *cind;  // column / block-column index
+    const void *val;       // values (blocks row-major r x c)
+    uint32_t nrows;        // rows (CSR/COO) or block rows (BCSR/BCOO)
+    uint32_t nent;         // entries or blocks
+    uint32_t nrows_elem;   // element rows (block formats: rows of the last block row may be cut)
+    uint32_t ncols_elem;   // element columns
+    int64_t xoff, yoff;    // element offsets into the per-call x / y bases
+};
+
+template <typename T> struct Ptrs {
+    const T *xb;
+    T *yb;
+    const uintptr_t *xtab;  // per-problem x pointers, or null
+    const uintptr_t *ytab;  // per-problem y pointers, or null
+    int ldx, ldy;
+    int q0;                 // first right-hand side of this launch (tables only)
+    __device__ __forceinline__ const T *x(uint32_t pid, const Prob &P) const {
+        return xtab ? (const T *)xtab[pid] + q0 : xb + P.xoff * ldx;
+    }
+    __device__ __forceinline__ T *y(uint32_t pid, const Prob &P) const {
+        return ytab ? (T *)ytab[pid] + q0 : yb + P.yoff * ldy;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// shared pieces
+// ---------------------------------------------------------------------------
+
+template <typename A> __device__ __forceinline__ A shfl_up_any(A v, int off) {
+    if constexpr (sizeof(A) == 4) {
+        return __builtin_bit_cast(A, __shfl_up(__builtin_bit_cast(int, v), off, WAVE));
+    } else {
+        uint64_t u = __builtin_bit_cast(uint64_t, v);
+        int lo = __shfl_up((int)(uint32_t)u, off, WAVE);
+        int hi = __shfl_up((int)(uint32_t)(u >> 32), off, WAVE);
+        return __builtin_bit_cast(A, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+}
+
+// Workgroup-wide inclusive segmented scan of (key, val[K]) over TPB threads;
+// keys are non-decreasing in thread order.  On return val holds the scan
+// and s_scan[q][tid] the same values for neighbours to read.
+template <typename A, int K>
+__device__ __forceinline__ void block_seg_scan(uint32_t key, A (&val)[K], uint32_t *s_key,
+                                               A (*s_scan)[TPB], A (*s_wc)[TPB / WAVE]) {
+    constexpr int NW = TPB / WAVE;
+    const int tid = threadIdx.x, lane = tid % WAVE, wid = tid / WAVE;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t ok = (uint32_t)__shfl_up((int)key, off, WAVE);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const A ov = shfl_up_any(val[q], off);
+            if (lane >= off && ok == key) val[q] = ov + val[q];
+        }
+    }
+    s_key[tid] = key;
+#pragma unroll
+    for (int q = 0; q < K; q++) s_scan[q][tid] = val[q];
+    __syncthreads();
+    if (tid < K) {
+        const int q = tid;
+        A run = A(0);
+        uint32_t run_key = NONE;
+        for (int w = 0; w < NW; w++) {
+            s_wc[q][w] = run;
+            const uint32_t lk = s_key[w * WAVE + WAVE - 1];
+            const A lv = s_scan[q][w * WAVE + WAVE - 1];
+            run = (w > 0 && lk == run_key) ? run + lv : lv;
+            run_key = lk;
+        }
+    }
+    __syncthreads();
+    if (wid > 0 && key == s_key[wid * WAVE - 1]) {
+#pragma unroll
+        for (int q = 0; q < K; q++) val[q] = s_wc[q][wid] + val[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; q++) s_scan[q][tid] = val[q];
+    __syncthreads();
+}
+
+template <typename RE>
+__device__ __host__ inline uint32_t merge_search(uint32_t d, uint32_t na, uint32_t nb, RE rowend,
+                                                 uint32_t jbase) {
+    uint32_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (rowend(mid) <= jbase + (d - mid - 1)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// CSR, entry balanced (merge path)
+// ---------------------------------------------------------------------------
+template <typename T, int IPT, int K>
+__global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__ probs,
+                                                        const uint32_t *__restrict__ unit_pid,
+                                                        const uint32_t *__restrict__ crow,
+                                                        const uint32_t *__restrict__ cnnz, Ptrs<T> io,
+                                                        uint32_t *__restrict__ carry_row,
+                                                        acc_t<T> *__restrict__ carry_val) {
+    using A = acc_t<T>;
+    constexpr int TILE = TPB * IPT;
+    // One right-hand side: stage products (the x gather happens in the
+    // coalesced load phase).  Several: stage (column, value) pairs and
+    // gather x rows during the walk.
+    constexpr bool PROD = (K == 1);
+    __shared__ uint32_t s_rowend[TILE + 1];
+    __shared__ A s_prod[PROD ? TILE : 1];
+    __shared__ uint32_t s_col[PROD ? 1 : TILE];
+    __shared__ T s_val[PROD ? 1 : TILE];
+    __shared__ uint32_t s_key[TPB];
+    __shared__ A s_scan[K][TPB];
+    __shared__ A s_wc[K][TPB / WAVE];
+
+    const int tid = threadIdx.x;
+    const uint32_t t = blockIdx.x, pid = unit_pid[t], ci = t + pid;
+    const Prob P = probs[pid];
+    const T *__restrict__ x = io.x(pid, P);
+    T *__restrict__ y = io.y(pid, P);
+    const T *__restrict__ vals = (const T *)P.val;
+    const int ldx = io.ldx, ldy = io.ldy;
+    const uint32_t i0 = crow[ci], j0 = cnnz[ci];
+    const uint32_t na = crow[ci + 1] - i0, nb = cnnz[ci + 1] - j0;
+
+    // All loads of the tile are issued before the first is consumed, then
+    // all x gathers: two memory round trips per tile.
+    const uint32_t nrl = min(na + 1, P.nrows - i0);
+    uint32_t re[IPT + 1], cv[IPT];
+    T vv[IPT];
+#pragma unroll
+    for (int q = 0; q <= IPT; q++) {
+        const uint32_t k = tid + q * TPB;
+        if (k < nrl) re[q] = P.ptr[i0 + k + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const uint32_t k = tid + q * TPB;
+        if (k < nb) {
+            cv[q] = ldg(P.cind + j0 + k);
+            vv[q] = ldg(vals + j0 + k);
+        }
+    }
+    if constexpr (PROD) {
+        T xv[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * TPB < nb) xv[q] = x[(size_t)cv[q] * ldx];
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * TPB < nb) s_prod[tid + q * TPB] = mul<T>(vv[q], xv[q]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * TPB < nb) {
+                s_col[tid + q * TPB] = cv[q];
+                s_val[tid + q * TPB] = vv[q];
+            }
+    }
+#pragma unroll
+    for (int q = 0; q <= IPT; q++)
+        if (tid + q * TPB < nrl) s_rowend[tid + q * TPB] = re[q];
+    __syncthreads();
+
+    const uint32_t total = na + nb;
+    const uint32_t d0 = min((uint32_t)tid * IPT, total), d1 = min(d0 + IPT, total);
+    uint32_t li = merge_search(d0, na, nb, [&](uint32_t i) { return s_rowend[i]; }, j0);
+    uint32_t lj = d0 - li;
+    A acc[K], first[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) { acc[q] = A(0); first[q] = A(0); }
+    bool emitted = false;
+    uint32_t first_row = 0;
+    for (uint32_t s = d0; s < d1; s++) {
+        if (li < na && s_rowend[li] <= j0 + lj) {
+            const uint32_t row = i0 + li;
+            if (!emitted) {
+                emitted = true;
+                first_row = row;
+#pragma unroll
+                for (int q = 0; q < K; q++) first[q] = acc[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < K; q++) y[(size_t)row * ldy + q] = to_val<T>(acc[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < K; q++) acc[q] = A(0);
+            li++;
+        } else {
+            if constexpr (PROD) {
+                acc[0] = acc[0] + s_prod[lj];
+            } else {
+                const uint32_t c = s_col[lj];
+                const T v = s_val[lj];
+#pragma unroll
+                for (int q = 0; q < K; q++) acc[q] = acc[q] + mul<T>(v, x[(size_t)c * ldx + q]);
+            }
+            lj++;
+        }
+    }
+    const uint32_t key = i0 + li;
+    block_seg_scan<A, K>(key, acc, s_key, s_scan, s_wc);
+    if (emitted) {
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const A cin = tid > 0 ? s_scan[q][tid - 1] : A(0);
+            y[(size_t)first_row * ldy + q] = to_val<T>(cin + first[q]);
+        }
+    }
+    if (tid == TPB - 1) {
+        carry_row[t] = key < P.nrows ? key : NONE;
+#pragma unroll
+        for (int q = 0; q < K; q++) carry_val[(size_t)t * K + q] = acc[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR, row granular: L lanes per row, rows of the unit interleaved over
+// the TPB/L lane groups.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int K>
+__global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict__ probs,
+                                                         const uint32_t *__restrict__ unit_pid,
+                                                         const uint32_t *__restrict__ urow, Ptrs<T> io) {
+    using A = acc_t<T>;
+    constexpr int G = TPB / L;
+    const uint32_t u = blockIdx.x, pid = unit_pid[u];
+    const Prob P = probs[pid];
+    const T *__restrict__ x = io.x(pid, P);
+    T *__restrict__ y = io.y(pid, P);
+    const T *__restrict__ vals = (const T *)P.val;
+    const uint32_t *__restrict__ colind = P.cind;
+    const int ldx = io.ldx, ldy = io.ldy;
+    const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    const int lane = threadIdx.x % L, grp = threadIdx.x / L;
+    for (uint32_t r = r0 + grp; r < r1; r += G) {
+        const uint32_t b = P.ptr[r], e = P.ptr[r + 1];
+        A acc[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) acc[q] = A(0);
+        uint32_t k = b + lane;
+        for (; k + L < e; k += 2 * L) {
+            const uint32_t c0 = ldg(colind + k), c1 = ldg(colind + k + L);
+            const T v0 = ldg(vals + k), v1 = ldg(vals + k + L);
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                acc[q] = fma_acc<T>(v0, x[(size_t)c0 * ldx + q], acc[q]);
+                acc[q] = fma_acc<T>(v1, x[(size_t)c1 * ldx + q], acc[q]);
+            }
+        }
+        if (k < e) {
+            const uint32_t c0 = ldg(colind + k);
+            const T v0 = ldg(vals + k);
+#pragma unroll
+            for (int q = 0; q < K; q++) acc[q] = fma_acc<T>(v0, x[(size_t)c0 * ldx + q], acc[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < K; q++) acc[q] = group_sum<L>(acc[q]);
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < K; q++) y[(size_t)r * ldy + q] = to_val<T>(acc[q]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// COO, entry balanced with a segmented reduction.  Entries are sorted by
+// row.  A thread owns IPT consecutive entries; a row is written by the
+// thread holding its last entry, with the partial sums of earlier threads
+// of the tile added in; partials of earlier tiles arrive in the fix-up.
+// ---------------------------------------------------------------------------
+template <typename T, int IPT, int K>
+__global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ probs,
+                                                      const uint32_t *__restrict__ unit_pid,
+                                                      const uint32_t *__restrict__ cent, Ptrs<T> io,
+                                                      uint32_t *__restrict__ carry_row,
+                                                      acc_t<T> *__restrict__ carry_val) {
+    using A = acc_t<T>;
+    constexpr int TILE = TPB * IPT;
+    constexpr bool PROD = (K == 1);
+    __shared__ uint32_t s_row[TILE];
+    __shared__ A s_prod[PROD ? TILE : 1];
+    __shared__ uint32_t s_col[PROD ? 1 : TILE];
+    __shared__ T s_val[PROD ? 1 : TILE];
+    __shared__ uint32_t s_key[TPB];
+    __shared__ A s_scan[K][TPB];
+    __shared__ A s_wc[K][TPB / WAVE];
+
+    const int tid = threadIdx.x;
+    const uint32_t t = blockIdx.x, pid = unit_pid[t], ci = t + pid;
+    const Prob P = probs[pid];
+    const T *__restrict__ x = io.x(pid, P);
+    T *__restrict__ y = io.y(pid, P);
+    const T *__restrict__ vals = (const T *)P.val;
+    const int ldx = io.ldx, ldy = io.ldy;
+    const uint32_t e0 = cent[ci], n = cent[ci + 1] - e0;
+    const uint32_t next_row = (e0 + n < P.nent) ? P.rind[e0 + n] : NONE;
+
+    uint32_t rv[IPT], cv[IPT];
+    T vv[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const uint32_t k = tid + q * TPB;
+        if (k < n) {
+            rv[q] = ldg(P.rind + e0 + k);
+            cv[q] = ldg(P.cind + e0 + k);
+            vv[q] = ldg(vals + e0 + k);
+        }
+    }
+    if constexpr (PROD) {
+        T xv[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * TPB < n) xv[q] = x[(size_t)cv[q] * ldx];
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * TPB < n) s_prod[tid + q * TPB] = mul<T>(vv[q], xv[q]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * TPB < n) {
+                s_col[tid + q * TPB] = cv[q];
+                s_val[tid + q * TPB] = vv[q];
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; q++)
+        if (tid + q * TPB < n) s_row[tid + q * TPB] = rv[q];
+    __syncthreads();
+
+    const uint32_t a = min((uint32_t)tid * IPT, n), b = min(a + IPT, n);
+    A acc[K], first[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) { acc[q] = A(0); first[q] = A(0); }
+    bool emitted = false, final_emitted = false;
+    uint32_t first_row = 0, cur = a < b ? s_row[a] : NONE;
+    for (uint32_t e = a; e < b; e++) {
+        const uint32_t r = s_row[e];
+        if (r != cur) {
+            if (!emitted) {
+                emitted = true;
+                first_row = cur;
+#pragma unroll
+                for (int q = 0; q < K; q++) first[q] = acc[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < K; q++) y[(size_t)cur * ldy + q] = to_val<T>(acc[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < K; q++) acc[q] = A(0);
+            cur = r;
+        }
+        if constexpr (PROD) {
+            acc[0] = acc[0] + s_prod[e];
+        } else {
+            const uint32_t c = s_col[e];
+            const T v = s_val[e];
+#pragma unroll
+            for (int q = 0; q < K; q++) acc[q] = acc[q] + mul<T>(v, x[(size_t)c * ldx + q]);
+        }
+    }
+    if (a < b) {
+        const uint32_t nxt = b < n ? s_row[b] : next_row;
+        if (nxt != cur) {  // this thread holds the row's last entry
+            final_emitted = true;
+            if (!emitted) {
+                emitted = true;
+                first_row = cur;
+#pragma unroll
+                for (int q = 0; q < K; q++) first[q] = acc[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < K; q++) y[(size_t)cur * ldy + q] = to_val<T>(acc[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < K; q++) acc[q] = A(0);
+        }
+    }
+    const uint32_t key = cur;
+    block_seg_scan<A, K>(key, acc, s_key, s_scan, s_wc);
+    if (emitted) {
+        const bool cont = tid > 0 && s_key[tid - 1] == first_row;
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const A cin = cont ? s_scan[q][tid - 1] : A(0);
+            y[(size_t)first_row * ldy + q] = to_val<T>(cin + first[q]);
+        }
+    }
+    const uint32_t last = n ? (n - 1) / IPT : 0;
+    if ((uint32_t)tid == last) {
+        carry_row[t] = (n == 0 || final_emitted) ? NONE : key;
+#pragma unroll
+        for (int q = 0; q < K; q++) carry_val[(size_t)t * K + q] = acc[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BCSR, register blocked: L lanes per block row; a lane takes whole R x C
+// blocks (contiguous values) and keeps R partial sums per right-hand side.
+// ---------------------------------------------------------------------------
+template <typename T, int R, int C, int L, int K>
+__global__ __launch_bounds__(TPB) void bcsr_vector_kernel(const Prob *__restrict__ probs,
+                                                          const uint32_t *__restrict__ unit_pid,
+                                                          const uint32_t *__restrict__ urow, Ptrs<T> io) {
+    using A = acc_t<T>;
+    constexpr int G = TPB / L;
+    const uint32_t u = blockIdx.x, pid = unit_pid[u];
+    const Prob P = probs[pid];
+    const T *__restrict__ x = io.x(pid, P);
+    T *__restrict__ y = io.y(pid, P);
+    const T *__restrict__ bv = (const T *)P.val;
+    const int ldx = io.ldx, ldy = io.ldy;
+    const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    const int lane = threadIdx.x % L, grp = threadIdx.x / L;
+    const uint32_t ncols = P.ncols_elem;
+    for (uint32_t I = r0 + grp; I < r1; I += G) {
+        const uint32_t b = P.ptr[I], e = P.ptr[I + 1];
+        A acc[R][K];
+#pragma unroll
+        for (int i = 0; i < R; i++)
+#pragma unroll
+            for (int q = 0; q < K; q++) acc[i][q] = A(0);
+        for (uint32_t k = b + lane; k < e; k += L) {
+            const uint32_t J = ldg(P.cind + k);
+            T v[R * C];
+#pragma unroll
+            for (int z = 0; z < R * C; z++) v[z] = ldg(bv + (size_t)k * (R * C) + z);
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const uint32_t col = J * C + c;
+                if (col < ncols) {
+#pragma unroll
+                    for (int q = 0; q < K; q++) {
+                        const T xv = x[(size_t)col * ldx + q];
+#pragma unroll
+                        for (int i = 0; i < R; i++) acc[i][q] = fma_acc<T>(v[i * C + c], xv, acc[i][q]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; i++)
+#pragma unroll
+            for (int q = 0; q < K; q++) acc[i][q] = group_sum<L>(acc[i][q]);
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const uint32_t row = I * R + i;
+                if (row < P.nrows_elem) {
+#pragma unroll
+                    for (int q = 0; q < K; q++) y[(size_t)row * ldy + q] = to_val<T>(acc[i][q]);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BCOO, block balanced: phase 1 forms the R row sums of every block of the
+// tile with lane groups of C over the block's elements (coalesced value
+// loads); phase 2 is the COO segmented walk over blocks with R-wide sums.
+// ---------------------------------------------------------------------------
+template <typename T, int R, int C, int IPT>
+__global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ probs,
+                                                       const uint32_t *__restrict__ unit_pid,
+                                                       const uint32_t *__restrict__ cent, Ptrs<T> io,
+                                                       uint32_t *__restrict__ carry_row,
+                                                       acc_t<T> *__restrict__ carry_val,
+                                                       uint32_t nunits) {
+    using A = acc_t<T>;
+    constexpr int TILE = TPB * IPT;  // blocks per tile
+    static_assert(TPB % (R * C) == 0, "block must tile the workgroup");
+    __shared__ uint32_t s_brow[TILE];
+    __shared__ uint32_t s_bcol[TILE];
+    __shared__ A s_part[TILE * R];
+    __shared__ uint32_t s_key[TPB];
+    __shared__ A s_scan[R][TPB];
+    __shared__ A s_wc[R][TPB / WAVE];
+
+    const int tid = threadIdx.x;
+    const uint32_t t = blockIdx.x, pid = unit_pid[t], ci = t + pid;
+    const Prob P = probs[pid];
+    const T *__restrict__ x = io.x(pid, P);
+    T *__restrict__ y = io.y(pid, P);
+    const T *__restrict__ bv = (const T *)P.val;
+    const int ldy = io.ldy, ldx = io.ldx;
+    const uint32_t b0 = cent[ci], n = cent[ci + 1] - b0;
+    const uint32_t next_brow = (b0 + n < P.nent) ? P.rind[b0 + n] : NONE;
+
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const uint32_t k = tid + q * TPB;
+        if (k < n) {
+            s_brow[k] = ldg(P.rind + b0 + k);
+            s_bcol[k] = ldg(P.cind + b0 + k);
+        }
+    }
+    __syncthreads();
+    // phase 1: element (blk, i, c) handled by thread tid + q*TPB
+    const uint32_t nel = n * (R * C);
+    for (uint32_t base = 0; base < (uint32_t)TILE * R * C; base += TPB) {
+        const uint32_t e = base + tid;
+        const uint32_t blk = e / (R * C), rem = e % (R * C), i = rem / C, c = rem % C;
+        A p = A(0);
+        if (e < nel) {
+            const uint32_t col = s_bcol[blk] * C + c;
+            if (col < P.ncols_elem) p = mul<T>(ldg(bv + (size_t)b0 * (R * C) + e), x[(size_t)col * ldx]);
+        }
+        // sum over the C lanes of this block row (fixed order: butterfly)
+        p = group_sum<C>(p);
+        if (c == 0 && e < nel) s_part[blk * R + i] = p;
+        if (base + TPB >= nel) break;
+    }
+    __syncthreads();
+
+    const uint32_t a = min((uint32_t)tid * IPT, n), bb = min(a + IPT, n);
+    A acc[R], first[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) { acc[i] = A(0); first[i] = A(0); }
+    bool emitted = false, final_emitted = false;
+    uint32_t first_row = 0, cur = a < bb ? s_brow[a] : NONE;
+    auto emit_block_row = [&](uint32_t brow, const A (&v)[R]) {
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const uint32_t row = brow * R + i;
+            if (row < P.nrows_elem) y[(size_t)row * ldy] = to_val<T>(v[i]);
+        }
+    };
+    for (uint32_t e = a; e < bb; e++) {
+        const uint32_t r = s_brow[e];
+        if (r != cur) {
+            if (!emitted) {
+                emitted = true;
+                first_row = cur;
+#pragma unroll
+                for (int i = 0; i < R; i++) first[i] = acc[i];
+            } else {
+                emit_block_row(cur, acc);
+            }
+#pragma unroll
+            for (int i = 0; i < R; i++) acc[i] = A(0);
+            cur = r;
+        }
+#pragma unroll
+        for (int i = 0; i < R; i++) acc[i] = acc[i] + s_part[e * R + i];
+    }
+    if (a < bb) {
+        const uint32_t nxt = bb < n ? s_brow[bb] : next_brow;
+        if (nxt != cur) {
+            final_emitted = true;
+            if (!emitted) {
+                emitted = true;
+                first_row = cur;
+#pragma unroll
+                for (int i = 0; i < R; i++) first[i] = acc[i];
+            } else {
+                emit_block_row(cur, acc);
+            }
+#pragma unroll
+            for (int i = 0; i < R; i++) acc[i] = A(0);
+        }
+    }
+    const uint32_t key = cur;
+    block_seg_scan<A, R>(key, acc, s_key, s_scan, s_wc);
+    if (emitted) {
+        const bool cont = tid > 0 && s_key[tid - 1] == first_row;
+        A v[R];
+#pragma unroll
+        for (int i = 0; i < R; i++) v[i] = (cont ? s_scan[i][tid - 1] : A(0)) + first[i];
+        emit_block_row(first_row, v);
+    }
+    const uint32_t last = n ? (n - 1) / IPT : 0;
+    if ((uint32_t)tid == last) {
+        const uint32_t kr = (n == 0 || final_emitted) ? NONE : key;
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            carry_row[(size_t)i * nunits + t] = kr;
+            carry_val[(size_t)i * nunits + t] = acc[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fix-up: threads [0, S*nunits) add carried partial sums of row runs that
+// cross tiles (slot s = row inside a block, S = 1 for element formats), in
+// tile order; threads beyond zero the listed rows that have no entries.
+// ---------------------------------------------------------------------------
+template <typename T, int K>
+__global__ void fixup_kernel(const Prob *__restrict__ probs, const uint32_t *__restrict__ unit_pid,
+                             const uint32_t *__restrict__ carry_row, const acc_t<T> *__restrict__ carry_val,
+                             uint32_t nunits, uint32_t S, Ptrs<T> io, const uint32_t *__restrict__ zpid,
+                             const uint32_t *__restrict__ zrow, uint32_t nzero) {
+    using A = acc_t<T>;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t ncarry = (uint64_t)S * nunits;
+    if (g < ncarry) {
+        const uint32_t s = (uint32_t)(g / nunits), t = (uint32_t)(g % nunits);
+        const uint32_t *cr = carry_row + (size_t)s * nunits;
+        const uint32_t r = cr[t];
+        if (r == NONE) return;
+        const uint32_t pid = unit_pid[t];
+        if (t > 0 && unit_pid[t - 1] == pid && cr[t - 1] == r) return;
+        A sum[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) sum[q] = A(0);
+        for (uint32_t u = t; u < nunits && unit_pid[u] == pid && cr[u] == r; u++) {
+#pragma unroll
+            for (int q = 0; q < K; q++) sum[q] = sum[q] + carry_val[((size_t)s * nunits + u) * K + q];
+        }
+        const Prob P = probs[pid];
+        const uint32_t row = r * S + s;
+        if (row >= P.nrows_elem) return;
+        T *yp = io.y(pid, P) + (size_t)row * io.ldy;
+#pragma unroll
+        for (int q = 0; q < K; q++) yp[q] = to_val<T>((A)yp[q] + sum[q]);
+        return;
+    }
+    const uint64_t z = g - ncarry;
+    if (z >= nzero) return;
+    const uint32_t pid = zpid[z];
+    const Prob P = probs[pid];
+    T *yp = io.y(pid, P) + (size_t)zrow[z] * io.ldy;
+#pragma unroll
+    for (int q = 0; q < K; q++) yp[q] = T(0);
+}
+
+}  // namespace spd

@@ -1,0 +1,129 @@
+// Typed runner for a Work: picks the kernel instantiation and launches the
+// main kernel, then the fix-up (carries + rows without entries) when the
+// kind needs one.  Included only by the per-dtype instantiation files.
+#pragma once
+
+#include "plan.hpp"
+#include "work.hpp"
+
+namespace spd {
+
+template <typename T, int K>
+static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
+    using A = acc_t<T>;
+    const dim3 grid(w.nunits), block(TPB);
+    A *cv = (A *)w.d_carry_val;
+    switch (w.kind) {
+    case WK_CSR_MERGE:
+        if (K > 1 && w.ipt > 8) return SP_ERR_UNSUPPORTED;
+        switch (w.ipt) {
+        case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv); break;
+        case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv); break;
+        case 16: hipLaunchKernelGGL((csr_merge_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv); break;
+        default: return SP_ERR_ARG;
+        }
+        break;
+    case WK_CSR_VEC:
+#define SPD_VEC(Lv) case Lv: hipLaunchKernelGGL((csr_vector_kernel<T, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io); break;
+        switch (w.lanes) {
+            SPD_VEC(1) SPD_VEC(2) SPD_VEC(4) SPD_VEC(8) SPD_VEC(16) SPD_VEC(32) SPD_VEC(64)
+        default: return SP_ERR_ARG;
+        }
+#undef SPD_VEC
+        break;
+    case WK_COO_SEG:
+        if (K > 1 && w.ipt > 8) return SP_ERR_UNSUPPORTED;
+        switch (w.ipt) {
+        case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv); break;
+        case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv); break;
+        case 16: hipLaunchKernelGGL((coo_seg_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv); break;
+        default: return SP_ERR_ARG;
+        }
+        break;
+    case WK_BCSR_VEC: {
+#define SPD_BL(Rv, Cv, Lv) \
+    if (w.R == Rv && w.C == Cv && w.lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io); break; }
+#define SPD_BRC(Rv, Cv) SPD_BL(Rv, Cv, 1) SPD_BL(Rv, Cv, 4) SPD_BL(Rv, Cv, 16) SPD_BL(Rv, Cv, 64)
+        do {
+            SPD_BRC(1, 1) SPD_BRC(2, 2) SPD_BRC(4, 4) SPD_BRC(8, 8) SPD_BRC(2, 4) SPD_BRC(4, 2)
+            return SP_ERR_UNSUPPORTED;
+        } while (0);
+#undef SPD_BRC
+#undef SPD_BL
+        break;
+    }
+    case WK_BCOO_SEG: {
+        if (K != 1) return SP_ERR_UNSUPPORTED;
+#define SPD_BO(Rv, Cv) \
+    if (w.R == Rv && w.C == Cv) { hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 4>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break; }
+        do {
+            SPD_BO(1, 1) SPD_BO(2, 2) SPD_BO(4, 4) SPD_BO(8, 8) SPD_BO(2, 4) SPD_BO(4, 2)
+            return SP_ERR_UNSUPPORTED;
+        } while (0);
+#undef SPD_BO
+        break;
+    }
+    default: return SP_ERR_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+}
+
+template <typename T, int K>
+static int launch_chunk(const Work &w, sp_ctx *ctx, Ptrs<T> io) {
+    hipStream_t s = ctx->stream;
+    if (w.nunits == 0 && w.nzero == 0) return SP_OK;
+    if (w.nunits) {
+        int rc = launch_main<T, K>(w, s, io);
+        if (rc) return rc;
+    }
+    mark_dominant(ctx);
+    const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG;
+    const uint32_t S = w.kind == WK_BCOO_SEG ? w.R : 1;
+    const uint64_t nthreads = (carries ? (uint64_t)S * w.nunits : 0) + w.nzero;
+    if (nthreads) {
+        using A = acc_t<T>;
+        const uint32_t nun = carries ? w.nunits : 0;
+        hipLaunchKernelGGL((fixup_kernel<T, K>), dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
+                           w.d_prob, w.d_unit_pid, w.d_carry_row, (const A *)w.d_carry_val, nun, S, io,
+                           w.d_zpid, w.d_zrow, w.nzero);
+        if (hipGetLastError() != hipSuccess) return SP_ERR_HIP;
+    }
+    return SP_OK;
+}
+
+}  // namespace spd
+
+template <typename T>
+int run_work_t(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintptr_t *xtab,
+               const uintptr_t *ytab, int k, int ldx, int ldy) {
+    using namespace spd;
+    int done = 0;
+    // Block COO kernels take one right-hand side per launch.
+    const int kmax = (w.kind == WK_BCOO_SEG) ? 1 : 8;
+    if (w.zero_all && w.zero_rows && yb) {
+        // Partial buffers of tiled plans: rows a tile has no entries for stay 0.
+        int rc = launch_fill_zero(ctx->stream, yb, (uint64_t)w.zero_rows * ldy * sizeof(T));
+        if (rc) return rc;
+    }
+    while (done < k) {
+        const int left = k - done;
+        int kc = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+        if (kc > kmax) kc = kmax;
+        if ((w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG) && (uint32_t)kc > w.carry_k) kc = (int)w.carry_k;
+        Ptrs<T> io;
+        io.xb = xb ? (const T *)xb + done : nullptr;
+        io.yb = yb ? (T *)yb + done : nullptr;
+        io.xtab = xtab;
+        io.ytab = ytab;
+        io.ldx = ldx;
+        io.ldy = ldy;
+        io.q0 = done;
+        int rc = kc == 8 ? launch_chunk<T, 8>(w, ctx, io)
+               : kc == 4 ? launch_chunk<T, 4>(w, ctx, io)
+               : kc == 2 ? launch_chunk<T, 2>(w, ctx, io)
+                         : launch_chunk<T, 1>(w, ctx, io);
+        if (rc) return rc;
+        done += kc;
+    }
+    return SP_OK;
+}

@@ -1,0 +1,458 @@
+// Plan builders: turn a host matrix plus the reference partitioners' splits
+// into the device Work of one variant.  SparseP's two levels map onto the
+// GPU as: part (ex DPU, `nparts`) -> a contiguous range of work units;
+// sub-part (ex tasklet, `nsub`) -> one work unit (a workgroup, or a run of
+// fixed-size tiles for the entry-balanced kernels).  The split arrays come
+// from the same partitioners the host surface exports.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "plan.hpp"
+
+using spd::merge_search;
+using spd::Prob;
+using spd::TPB;
+
+namespace {
+
+size_t acc_size(int dt) { return (dt == SP_I64 || dt == SP_F64) ? 8 : 4; }
+
+struct Builder {
+    sp_plan *p;
+    std::vector<Prob> probs;
+    std::vector<uint32_t> unit_pid, c0, c1;
+    std::vector<uint32_t> zpid, zrow;
+
+    explicit Builder(sp_plan *pl) : p(pl) {}
+
+    void add(const Prob &P, const std::vector<uint32_t> &k0, const std::vector<uint32_t> *k1) {
+        const uint32_t pid = (uint32_t)probs.size();
+        probs.push_back(P);
+        const size_t nu = k0.empty() ? 0 : k0.size() - 1;
+        for (size_t u = 0; u < nu; u++) unit_pid.push_back(pid);
+        if (k0.empty()) c0.push_back(0);
+        else c0.insert(c0.end(), k0.begin(), k0.end());
+        if (k1) {
+            if (k1->empty()) c1.push_back(0);
+            else c1.insert(c1.end(), k1->begin(), k1->end());
+        }
+    }
+
+    int finish(int kind, bool carries, uint32_t slots) {
+        Work &w = p->work;
+        w.kind = kind;
+        w.dtype = p->dtype;
+        w.nprob = (uint32_t)probs.size();
+        w.nunits = (uint32_t)unit_pid.size();
+        int rc;
+        if ((rc = plan_upload(p, probs.data(), probs.size() * sizeof(Prob), (void **)&w.d_prob))) return rc;
+        if ((rc = plan_upload(p, unit_pid.data(), unit_pid.size() * 4, (void **)&w.d_unit_pid))) return rc;
+        if ((rc = plan_upload(p, c0.data(), c0.size() * 4, (void **)&w.d_c0))) return rc;
+        if (!c1.empty() && (rc = plan_upload(p, c1.data(), c1.size() * 4, (void **)&w.d_c1))) return rc;
+        w.nzero = (uint32_t)zrow.size();
+        if (w.nzero) {
+            if ((rc = plan_upload(p, zpid.data(), zpid.size() * 4, (void **)&w.d_zpid))) return rc;
+            if ((rc = plan_upload(p, zrow.data(), zrow.size() * 4, (void **)&w.d_zrow))) return rc;
+        }
+        if (carries) {
+            w.carry_k = kind == WK_BCOO_SEG ? 1 : 8;
+            const size_t n = (size_t)std::max<uint32_t>(w.nunits, 1) * slots;
+            if ((rc = plan_alloc(p, n * 4, (void **)&w.d_carry_row))) return rc;
+            if ((rc = plan_alloc(p, n * w.carry_k * acc_size(p->dtype), &w.d_carry_val))) return rc;
+        }
+        p->launches = 1 + ((carries || w.nzero) ? 1 : 0);
+        return SP_OK;
+    }
+};
+
+// Contiguous, strictly increasing boundaries from a list of cut points.
+void push_cut(std::vector<uint32_t> &v, uint32_t c) {
+    if (v.empty() || c > v.back()) v.push_back(c);
+}
+
+// Merge-path tile coordinates of a CSR problem (tile = TPB*ipt items).
+void merge_coords(const uint32_t *rp, uint32_t nrows, uint32_t nnz, uint32_t tile, std::vector<uint32_t> &crow,
+                  std::vector<uint32_t> &cnnz) {
+    const uint64_t total = (uint64_t)nrows + nnz;
+    const uint32_t nt = (uint32_t)((total + tile - 1) / tile);
+    crow.resize(nt + 1);
+    cnnz.resize(nt + 1);
+    const uint32_t base = rp[0];
+    for (uint32_t t = 0; t <= nt; t++) {
+        const uint64_t d = std::min<uint64_t>((uint64_t)t * tile, total);
+        const uint32_t i = merge_search((uint32_t)d, nrows, nnz, [&](uint32_t r) { return rp[r + 1] - base; }, 0);
+        crow[t] = i;
+        cnnz[t] = (uint32_t)(d - i) + base;
+    }
+}
+
+// Row units of one row range [r0, r1) of a CSR (or BCSR block-row) pointer
+// array: the tasklet-level split of the reference (by rows or by entries,
+// with its row granularity), one unit per tasklet.
+int row_units(const uint32_t *rp, uint32_t r0, uint32_t r1, uint32_t ntasklets, int by_nnz, uint32_t gran,
+              std::vector<uint32_t> &out) {
+    const uint32_t rows = r1 - r0;
+    if (rows == 0) return SP_OK;
+    std::vector<uint32_t> split(ntasklets + 1);
+    int rc = by_nnz ? sp_partition_tsklt_by_nnz(rp, r0, rows, rp[r1] - rp[r0], ntasklets, gran, split.data())
+                    : sp_partition_tsklt_by_row(rows, ntasklets, gran, split.data());
+    if (rc) return rc;
+    split[ntasklets] = rows;  // the reference can leave trailing rows uncovered; cover them
+    for (uint32_t i = 0; i <= ntasklets; i++) push_cut(out, r0 + std::min(split[i], rows));
+    return SP_OK;
+}
+
+uint32_t auto_lanes(double avg) {
+    uint32_t L = 1;
+    while (L < 64 && L * 2 <= avg * 1.5) L *= 2;
+    return L;
+}
+
+uint32_t pick_ipt(const sp_opts &o, uint32_t def) {
+    const uint32_t I = o.items_per_lane ? o.items_per_lane : def;
+    return (I == 4 || I == 8 || I == 16) ? I : 0;
+}
+
+// Tile an entry range [e0, e1) into fixed tiles of at most `tile` entries.
+void tile_range(uint32_t e0, uint32_t e1, uint32_t tile, std::vector<uint32_t> &out) {
+    push_cut(out, e0);
+    for (uint64_t e = (uint64_t)e0 + tile; e < e1; e += tile) push_cut(out, (uint32_t)e);
+    push_cut(out, e1);
+}
+
+bool coo_sorted(const sp_coo *A) {
+    for (uint32_t k = 1; k < A->nnz; k++) {
+        if (A->rowind[k] < A->rowind[k - 1]) return false;
+        if (A->rowind[k] == A->rowind[k - 1] && A->colind[k] < A->colind[k - 1]) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// CSR (ids 0, 1)
+// ---------------------------------------------------------------------------
+int build_csr_plan(sp_plan *p, const sp_csr *A) {
+    const size_t sv = sp_dtype_size(A->dtype);
+    if (!A->rowptr || (A->nnz && (!A->colind || !A->values))) return SP_ERR_ARG;
+    if (A->rowptr[0] != 0 || A->rowptr[A->nrows] != A->nnz) return SP_ERR_ARG;
+    for (uint32_t i = 0; i < A->nrows; i++)
+        if (A->rowptr[i + 1] < A->rowptr[i]) return SP_ERR_ARG;
+    for (uint32_t k = 0; k < A->nnz; k++)
+        if (A->colind[k] >= A->ncols) return SP_ERR_ARG;
+    if (p->algo == SP_ALGO_AUTO) {
+        uint32_t maxlen = 0;
+        for (uint32_t i = 0; i < A->nrows; i++) maxlen = std::max(maxlen, A->rowptr[i + 1] - A->rowptr[i]);
+        const double avg = A->nrows ? (double)A->nnz / A->nrows : 0.0;
+        p->algo = (maxlen <= 4 * avg + 64) ? SP_ALGO_VECTOR : SP_ALGO_SEGMENTED;
+    }
+    Prob P{};
+    int rc;
+    void *d;
+    if ((rc = plan_upload(p, A->rowptr, ((size_t)A->nrows + 1) * 4, &d))) return rc;
+    P.ptr = (const uint32_t *)d;
+    if ((rc = plan_upload(p, A->colind, (size_t)A->nnz * 4, &d))) return rc;
+    P.cind = (const uint32_t *)d;
+    if ((rc = plan_upload(p, A->values, (size_t)A->nnz * sv, &d))) return rc;
+    P.val = d;
+    P.nrows = P.nrows_elem = A->nrows;
+    P.nent = A->nnz;
+    P.ncols_elem = A->ncols;
+    Builder b(p);
+    if (p->algo == SP_ALGO_VECTOR) {
+        const uint32_t L = p->opts.lanes_per_row ? p->opts.lanes_per_row
+                                                 : auto_lanes(A->nrows ? (double)A->nnz / A->nrows : 1.0);
+        if (L > 64 || (L & (L - 1))) return SP_ERR_ARG;
+        p->lanes = p->work.lanes = L;
+        const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
+        std::vector<uint32_t> split(P1 + 1);
+        rc = p->id == SP_K_CSR_NNZ ? sp_partition_by_nnz(A->rowptr, A->nrows, P1, split.data())
+                                   : sp_partition_by_row(A->nrows, P1, split.data());
+        if (rc) return rc;
+        split[P1] = A->nrows;
+        const uint32_t gran = (uint32_t)(8 / sv);
+        const uint64_t rows_per_unit = 4ULL * (TPB / L);
+        std::vector<uint32_t> units;
+        push_cut(units, 0);
+        for (uint32_t q = 0; q < P1; q++) {
+            const uint32_t r0 = split[q], r1 = std::max(split[q + 1], r0);
+            const uint32_t T = p->opts.nsub ? p->opts.nsub
+                                            : (uint32_t)std::max<uint64_t>(1, (r1 - r0 + rows_per_unit - 1) / rows_per_unit);
+            if ((rc = row_units(A->rowptr, r0, r1, T, p->opts.sub_balance == SP_BAL_NNZ, gran, units))) return rc;
+        }
+        push_cut(units, A->nrows);
+        if (A->nrows == 0) units.clear();
+        p->nparts = P1;
+        p->nsub = units.empty() ? 0 : (uint32_t)units.size() - 1;
+        b.add(P, units, nullptr);
+        return b.finish(WK_CSR_VEC, false, 1);
+    }
+    const uint32_t I = pick_ipt(p->opts, 8);
+    if (!I) return SP_ERR_ARG;
+    p->ipt = p->work.ipt = I;
+    std::vector<uint32_t> crow, cnnz;
+    if (A->nrows) merge_coords(A->rowptr, A->nrows, A->nnz, TPB * I, crow, cnnz);
+    p->nparts = p->ntiles = crow.empty() ? 0 : (uint32_t)crow.size() - 1;
+    p->nsub = TPB;
+    b.add(P, crow, &cnnz);
+    return b.finish(WK_CSR_MERGE, true, 1);
+}
+
+// ---------------------------------------------------------------------------
+// COO (ids 2, 3, 4)
+// ---------------------------------------------------------------------------
+int build_coo_plan(sp_plan *p, const sp_coo *A0) {
+    const sp_coo *A = A0;
+    sp_coo *sorted = nullptr;
+    for (uint32_t k = 0; k < A0->nnz; k++)
+        if (A0->rowind[k] >= A0->nrows || A0->colind[k] >= A0->ncols) return SP_ERR_ARG;
+    if (!coo_sorted(A0)) {  // every COO app.c sorts first (spmv/1D/COO-row/host/app.c calls sortCOOMatrix)
+        int rc = sp_coo_alloc(A0->nrows, A0->ncols, A0->nnz, A0->dtype, &sorted);
+        if (rc) return rc;
+        memcpy(sorted->rowind, A0->rowind, (size_t)A0->nnz * 4);
+        memcpy(sorted->colind, A0->colind, (size_t)A0->nnz * 4);
+        memcpy(sorted->values, A0->values, (size_t)A0->nnz * sp_dtype_size(A0->dtype));
+        if ((rc = sp_sort_coo(sorted))) { sp_free_coo(sorted); return rc; }
+        A = sorted;
+    }
+    const size_t sv = sp_dtype_size(A->dtype);
+    const uint32_t n = A->nrows, nnz = A->nnz;
+    std::vector<uint32_t> rowptr(n + 1, 0);
+    for (uint32_t k = 0; k < nnz; k++) rowptr[A->rowind[k] + 1]++;
+    for (uint32_t i = 0; i < n; i++) rowptr[i + 1] += rowptr[i];
+    Prob P{};
+    int rc = SP_OK;
+    void *d;
+    Builder b(p);
+    do {
+        if ((rc = plan_upload(p, A->rowind, (size_t)nnz * 4, &d))) break;
+        P.rind = (const uint32_t *)d;
+        if ((rc = plan_upload(p, A->colind, (size_t)nnz * 4, &d))) break;
+        P.cind = (const uint32_t *)d;
+        if ((rc = plan_upload(p, A->values, (size_t)nnz * sv, &d))) break;
+        P.val = d;
+        P.nrows = P.nrows_elem = n;
+        P.nent = nnz;
+        P.ncols_elem = A->ncols;
+        const uint32_t I = pick_ipt(p->opts, 8);
+        if (!I) { rc = SP_ERR_ARG; break; }
+        p->ipt = p->work.ipt = I;
+        p->algo = SP_ALGO_SEGMENTED;
+        const uint32_t tile = TPB * I;
+        const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
+        // part entry ranges
+        std::vector<uint32_t> pe0(P1), pe1(P1);
+        if (p->id == SP_K_COO_NNZ) {
+            std::vector<uint32_t> prow(P1), rpp(P1), merge(P1);
+            std::vector<uint32_t> rows_nnz(n);
+            for (uint32_t i = 0; i < n; i++) rows_nnz[i] = rowptr[i + 1] - rowptr[i];
+            if (nnz == 0) {
+                for (uint32_t q = 0; q < P1; q++) pe0[q] = pe1[q] = 0;
+            } else {
+                std::vector<uint32_t> cnt(P1);
+                if ((rc = sp_partition_coo_nnz(rows_nnz.data(), n, nnz, P1, prow.data(), rpp.data(), pe0.data(),
+                                               cnt.data(), merge.data())))
+                    break;
+                for (uint32_t q = 0; q < P1; q++) pe1[q] = pe0[q] + cnt[q];
+            }
+        } else {
+            std::vector<uint32_t> split(P1 + 1);
+            rc = p->id == SP_K_COO_NNZ_RGRN ? sp_partition_by_nnz(rowptr.data(), n, P1, split.data())
+                                            : sp_partition_by_row(n, P1, split.data());
+            if (rc) break;
+            split[P1] = n;
+            for (uint32_t q = 0; q < P1; q++) {
+                pe0[q] = rowptr[split[q]];
+                pe1[q] = rowptr[std::max(split[q + 1], split[q])];
+            }
+        }
+        // sub-parts: the reference tasklet split of each part, then fixed tiles
+        std::vector<uint32_t> units;
+        push_cut(units, 0);
+        for (uint32_t q = 0; q < P1; q++) {
+            const uint32_t T = p->opts.nsub ? p->opts.nsub : 1;
+            const uint32_t cnt = pe1[q] - pe0[q];
+            if (cnt == 0) continue;
+            std::vector<uint32_t> st(T), ct(T);
+            if ((rc = sp_partition_even(cnt, T, st.data(), ct.data()))) break;
+            for (uint32_t s = 0; s < T; s++)
+                if (ct[s]) tile_range(pe0[q] + st[s], pe0[q] + st[s] + ct[s], tile, units);
+        }
+        if (rc) break;
+        push_cut(units, nnz);
+        if (nnz == 0) units.clear();
+        p->nparts = P1;
+        p->ntiles = units.empty() ? 0 : (uint32_t)units.size() - 1;
+        for (uint32_t i = 0; i < n; i++)
+            if (rowptr[i + 1] == rowptr[i]) { b.zpid.push_back(0); b.zrow.push_back(i); }
+        b.add(P, units, nullptr);
+        rc = b.finish(WK_COO_SEG, true, 1);
+    } while (0);
+    if (sorted) sp_free_coo(sorted);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// BCSR (ids 5, 6)
+// ---------------------------------------------------------------------------
+int build_bcsr_plan(sp_plan *p, const sp_bcsr *B) {
+    const size_t sv = sp_dtype_size(B->dtype);
+    const uint32_t R = B->row_block_size, C = B->col_block_size, nbr = B->num_block_rows;
+    const size_t rc2 = (size_t)R * C;
+    Prob P{};
+    int rc;
+    void *d;
+    if ((rc = plan_upload(p, B->browptr, ((size_t)nbr + 1) * 4, &d))) return rc;
+    P.ptr = (const uint32_t *)d;
+    if ((rc = plan_upload(p, B->bcolind, B->num_blocks * 4, &d))) return rc;
+    P.cind = (const uint32_t *)d;
+    if ((rc = plan_upload(p, B->bval, B->num_blocks * rc2 * sv, &d))) return rc;
+    P.val = d;
+    P.nrows = nbr;
+    P.nent = (uint32_t)B->num_blocks;
+    P.nrows_elem = B->nrows;
+    P.ncols_elem = B->ncols;
+    p->algo = SP_ALGO_VECTOR;
+    uint32_t L = p->opts.lanes_per_row;
+    if (L == 0) {
+        const double avg = nbr ? (double)B->num_blocks / nbr : 1.0;
+        L = avg >= 48 ? 64 : avg >= 12 ? 16 : avg >= 3 ? 4 : 1;
+    }
+    if (L != 1 && L != 4 && L != 16 && L != 64) return SP_ERR_ARG;
+    p->lanes = p->work.lanes = L;
+    p->work.R = R;
+    p->work.C = C;
+    const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
+    std::vector<uint32_t> bs(P1 + 1), bl(P1), ns(P1);
+    if ((rc = sp_partition_bcsr(B->browptr, B->nnz_per_block, nbr, B->num_blocks, B->nnz, P1,
+                                p->id == SP_K_BCSR_NNZ, bs.data(), bl.data(), ns.data())))
+        return rc;
+    bs[P1] = nbr;
+    const uint64_t brows_per_unit = 4ULL * (TPB / L);
+    std::vector<uint32_t> units;
+    push_cut(units, 0);
+    for (uint32_t q = 0; q < P1; q++) {
+        const uint32_t r0 = bs[q], r1 = std::max(bs[q + 1], r0);
+        if (r1 == r0) continue;
+        const uint32_t T = p->opts.nsub ? p->opts.nsub
+                                        : (uint32_t)std::max<uint64_t>(1, (r1 - r0 + brows_per_unit - 1) / brows_per_unit);
+        std::vector<uint32_t> ts(T + 1);
+        if ((rc = sp_partition_tsklt_bcsr(B->browptr, B->nnz_per_block, bs.data(), bl.data(), ns.data(), q, T,
+                                          p->opts.sub_balance == SP_BAL_NNZ, ts.data())))
+            return rc;
+        ts[T] = r1;
+        for (uint32_t i = 0; i <= T; i++) push_cut(units, std::min(std::max(ts[i], r0), r1));
+    }
+    push_cut(units, nbr);
+    if (nbr == 0) units.clear();
+    p->nparts = P1;
+    p->nsub = units.empty() ? 0 : (uint32_t)units.size() - 1;
+    Builder b(p);
+    b.add(P, units, nullptr);
+    return b.finish(WK_BCSR_VEC, false, 1);
+}
+
+// ---------------------------------------------------------------------------
+// BCOO (ids 7, 8)
+// ---------------------------------------------------------------------------
+int build_bcoo_plan(sp_plan *p, const sp_bcoo *B) {
+    const size_t sv = sp_dtype_size(B->dtype);
+    const uint32_t R = B->row_block_size, C = B->col_block_size;
+    const size_t rc2 = (size_t)R * C;
+    const uint64_t nb = B->num_blocks;
+    if (nb > 0xFFFFFFFFULL) return SP_ERR_UNSUPPORTED;
+    Prob P{};
+    int rc;
+    void *d;
+    if ((rc = plan_upload(p, B->browind, nb * 4, &d))) return rc;
+    P.rind = (const uint32_t *)d;
+    if ((rc = plan_upload(p, B->bcolind, nb * 4, &d))) return rc;
+    P.cind = (const uint32_t *)d;
+    if ((rc = plan_upload(p, B->bval, nb * rc2 * sv, &d))) return rc;
+    P.val = d;
+    P.nrows = B->num_block_rows;
+    P.nent = (uint32_t)nb;
+    P.nrows_elem = B->nrows;
+    P.ncols_elem = B->ncols;
+    p->algo = SP_ALGO_SEGMENTED;
+    p->work.R = R;
+    p->work.C = C;
+    p->ipt = p->work.ipt = 4;
+    const uint32_t tile = TPB * 4;
+    const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
+    std::vector<uint32_t> bs(P1 + 1), ns(P1);
+    if ((rc = sp_partition_bcoo(B->nnz_per_block, nb, B->nnz, P1, p->id == SP_K_BCOO_NNZ, bs.data(), ns.data())))
+        return rc;
+    bs[P1] = (uint32_t)nb;
+    std::vector<uint32_t> units;
+    push_cut(units, 0);
+    for (uint32_t q = 0; q < P1; q++) {
+        const uint32_t b0 = bs[q], b1 = std::max(bs[q + 1], b0);
+        if (b1 == b0) continue;
+        const uint32_t T = p->opts.nsub ? p->opts.nsub : 1;
+        std::vector<uint32_t> ts(T + 1);
+        if ((rc = sp_partition_tsklt_bcoo(B->nnz_per_block, bs.data(), ns.data(), q, T,
+                                          p->opts.sub_balance == SP_BAL_NNZ, ts.data())))
+            return rc;
+        ts[T] = b1;
+        for (uint32_t i = 0; i < T; i++) {
+            const uint32_t s0 = std::min(std::max(ts[i], b0), b1), s1 = std::min(std::max(ts[i + 1], s0), b1);
+            if (s1 > s0) tile_range(s0, s1, tile, units);
+        }
+    }
+    push_cut(units, (uint32_t)nb);
+    if (nb == 0) units.clear();
+    p->nparts = P1;
+    p->ntiles = units.empty() ? 0 : (uint32_t)units.size() - 1;
+    Builder b(p);
+    // element rows of block rows without blocks
+    std::vector<uint8_t> has(B->num_block_rows, 0);
+    for (uint64_t k = 0; k < nb; k++) has[B->browind[k]] = 1;
+    for (uint32_t I = 0; I < B->num_block_rows; I++)
+        if (!has[I])
+            for (uint32_t i = 0; i < R && (uint64_t)I * R + i < B->nrows; i++) {
+                b.zpid.push_back(0);
+                b.zrow.push_back(I * R + i);
+            }
+    b.add(P, units, nullptr);
+    return b.finish(WK_BCOO_SEG, true, R);
+}
+
+// ---------------------------------------------------------------------------
+// Execution
+// ---------------------------------------------------------------------------
+int run_plan(sp_plan *p, const void *x, void *y, int k, int ldx, int ldy) {
+    if (p->nrows == 0) return SP_OK;
+    if (!p->tiled) return run_work(p->work, p->ctx, x, y, nullptr, nullptr, k, ldx, ldy);
+    // 2D: partial rows per stripe into ypart, then the stripe-order reduction.
+    const uint64_t need = (uint64_t)p->vert * p->nrows * (uint64_t)k * sp_dtype_size(p->dtype);
+    if (need > p->ypart_bytes) {
+        void *d = nullptr;
+        int rc = plan_alloc(p, need, &d);
+        if (rc) return rc;
+        p->d_ypart = d;
+        p->ypart_bytes = need;
+    }
+    Work w = p->work;
+    w.zero_rows = w.zero_all ? (uint64_t)p->vert * p->nrows : 0;
+    int rc = run_work(w, p->ctx, x, p->d_ypart, nullptr, nullptr, k, ldx, k);
+    if (rc) return rc;
+    return launch_reduce_parts(p->ctx->stream, p->dtype, p->d_ypart, p->vert, (uint64_t)p->nrows * k, y,
+                               p->nrows, k, ldy);
+}
+
+int run_work(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintptr_t *xtab, const uintptr_t *ytab,
+             int k, int ldx, int ldy) {
+    switch (w.dtype) {
+    case SP_I8: return run_work_t<int8_t>(w, ctx, xb, yb, xtab, ytab, k, ldx, ldy);
+    case SP_I16: return run_work_t<int16_t>(w, ctx, xb, yb, xtab, ytab, k, ldx, ldy);
+    case SP_I32: return run_work_t<int32_t>(w, ctx, xb, yb, xtab, ytab, k, ldx, ldy);
+    case SP_I64: return run_work_t<int64_t>(w, ctx, xb, yb, xtab, ytab, k, ldx, ldy);
+    case SP_F32: return run_work_t<float>(w, ctx, xb, yb, xtab, ytab, k, ldx, ldy);
+    case SP_F64: return run_work_t<double>(w, ctx, xb, yb, xtab, ytab, k, ldx, ldy);
+    default: return SP_ERR_ARG;
+    }
+}

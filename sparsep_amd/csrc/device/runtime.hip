@@ -147,8 +147,7 @@ int plan_upload(sp_plan *p, const void *host, size_t bytes, void **out) {
 
 static void plan_free(sp_plan *p) {
     if (!p) return;
-    for (sp_plan *s : p->sub) plan_free(s);
-    for (void *d : p->allocs) hipFree(d);
+    for (void *d : p->allocs) (void)hipFree(d);
     delete p;
 }
 
@@ -170,30 +169,21 @@ static sp_plan *new_plan(sp_ctx *ctx, int id, int dtype, const sp_opts *opts) {
     else sp_opts_default(&p->opts);
     if (p->opts.row_block_size == 0) p->opts.row_block_size = 4;
     if (p->opts.col_block_size == 0) p->opts.col_block_size = 4;
+    p->algo = p->opts.algo;
     return p;
 }
-
-int tiled_plan_create(sp_ctx *ctx, int id, const sp_coo *A, sp_plan *p);  // tiled.cpp
 
 extern "C" int sp_plan_create_csr(sp_ctx *ctx, int id, const sp_csr *A, const sp_opts *opts,
                                   sp_plan **out) {
     if (!ctx || !A || !out || !is_csr(id)) return SP_ERR_ARG;
     if (sp_dtype_size(A->dtype) == 0) return SP_ERR_ARG;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     sp_plan *p = new_plan(ctx, id, A->dtype, opts);
     p->nrows = A->nrows;
     p->ncols = A->ncols;
     p->nnz = A->nnz;
-    p->algo = p->opts.algo;
-    if (p->algo == SP_ALGO_AUTO) {
-        // Row-granular lanes for regular row lengths, merge path otherwise.
-        uint32_t maxlen = 0;
-        for (uint32_t i = 0; i < A->nrows; i++) maxlen = std::max(maxlen, A->rowptr[i + 1] - A->rowptr[i]);
-        const double avg = A->nrows ? (double)A->nnz / A->nrows : 0.0;
-        p->algo = (maxlen <= 4 * avg + 64) ? SP_ALGO_VECTOR : SP_ALGO_SEGMENTED;
-    }
     p->algo_bytes = csr_bytes(A->nrows, A->ncols, A->nnz, sp_dtype_size(A->dtype));
-    int rc = csr_setup(p, A);
+    int rc = build_csr_plan(p, A);
     if (rc) { plan_free(p); return rc; }
     *out = p;
     return SP_OK;
@@ -202,7 +192,10 @@ extern "C" int sp_plan_create_csr(sp_ctx *ctx, int id, const sp_csr *A, const sp
 extern "C" int sp_plan_create(sp_ctx *ctx, int id, const sp_coo *A, const sp_opts *opts, sp_plan **out) {
     if (!ctx || !A || !out || id < 0 || id >= SP_K_COUNT) return SP_ERR_ARG;
     if (sp_dtype_size(A->dtype) == 0) return SP_ERR_ARG;
-    hipSetDevice(ctx->device);
+    if (A->nnz && (!A->rowind || !A->colind || !A->values)) return SP_ERR_ARG;
+    for (uint32_t k = 0; k < A->nnz; k++)
+        if (A->rowind[k] >= A->nrows || A->colind[k] >= A->ncols) return SP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
     if (is_csr(id)) {
         sp_csr *C = nullptr;
         int rc = sp_coo2csr(A, &C);
@@ -215,12 +208,11 @@ extern "C" int sp_plan_create(sp_ctx *ctx, int id, const sp_coo *A, const sp_opt
     p->nrows = A->nrows;
     p->ncols = A->ncols;
     p->nnz = A->nnz;
-    p->algo = p->opts.algo;
     const uint64_t sv = sp_dtype_size(A->dtype);
     int rc = SP_OK;
     if (is_coo(id)) {
         p->algo_bytes = (uint64_t)A->nnz * (8 + sv) + ((uint64_t)A->nrows + A->ncols) * sv;
-        rc = coo_setup(p, A);
+        rc = build_coo_plan(p, A);
     } else if (is_bcsr(id) || is_bcoo(id)) {
         sp_csr *C = nullptr;
         sp_bcsr *B = nullptr;
@@ -234,18 +226,18 @@ extern "C" int sp_plan_create(sp_ctx *ctx, int id, const sp_coo *A, const sp_opt
             if (is_bcsr(id)) {
                 p->algo_bytes = B->num_blocks * (rc2 * sv + 4) + ((uint64_t)B->num_block_rows + 1) * 4 +
                                 ((uint64_t)A->nrows + A->ncols) * sv;
-                rc = bcsr_setup(p, B);
+                rc = build_bcsr_plan(p, B);
             } else {
                 sp_bcoo *O = nullptr;
                 rc = sp_bcsr2bcoo(B, &O);
                 p->algo_bytes = B->num_blocks * (rc2 * sv + 8) + ((uint64_t)A->nrows + A->ncols) * sv;
-                if (!rc) rc = bcoo_setup(p, O);
+                if (!rc) rc = build_bcoo_plan(p, O);
                 sp_free_bcoo(O);
             }
         }
         sp_free_bcsr(B);
     } else {
-        rc = tiled_plan_create(ctx, id, A, p);
+        rc = build_tiled_plan(p, A);
     }
     if (rc) { plan_free(p); return rc; }
     *out = p;
@@ -254,8 +246,8 @@ extern "C" int sp_plan_create(sp_ctx *ctx, int id, const sp_coo *A, const sp_opt
 
 extern "C" int sp_plan_destroy(sp_plan *p) {
     if (!p) return SP_ERR_ARG;
-    hipSetDevice(p->ctx->device);
-    hipStreamSynchronize(p->ctx->stream);
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
     plan_free(p);
     return SP_OK;
 }
@@ -271,9 +263,7 @@ extern "C" int sp_plan_get_info(const sp_plan *p, sp_plan_info *i) {
     i->nnz = p->nnz;
     i->num_blocks = p->nb;
     i->algo_bytes = p->algo_bytes;
-    uint64_t dev = p->device_bytes;
-    for (const sp_plan *s : p->sub) dev += s->device_bytes;
-    i->device_bytes = dev;
+    i->device_bytes = p->device_bytes;
     i->nparts = p->nparts;
     i->nsub = p->nsub;
     i->lanes_per_row = p->lanes;
@@ -282,18 +272,12 @@ extern "C" int sp_plan_get_info(const sp_plan *p, sp_plan_info *i) {
     i->ntiles = p->ntiles;
     i->row_block_size = p->opts.row_block_size;
     i->col_block_size = p->opts.col_block_size;
-    i->launches = p->launches;
+    i->launches = p->launches + (p->tiled ? 1 : 0);
     return SP_OK;
 }
 
-int tiled_run(sp_plan *p, const void *x, void *y, int k, int ldx, int ldy);  // tiled.cpp
-
 static int plan_run(sp_plan *p, const void *x, void *y, int k, int ldx, int ldy) {
-    if (is_csr(p->id)) return csr_run(p, x, y, k, ldx, ldy);
-    if (is_coo(p->id)) return coo_run(p, x, y, k, ldx, ldy);
-    if (is_bcsr(p->id)) return bcsr_run(p, x, y, k, ldx, ldy);
-    if (is_bcoo(p->id)) return bcoo_run(p, x, y, k, ldx, ldy);
-    return tiled_run(p, x, y, k, ldx, ldy);
+    return run_plan(p, x, y, k, ldx, ldy);
 }
 
 extern "C" int sp_spmv(sp_plan *p, const void *x, void *y) {

@@ -34,15 +34,16 @@ __global__ void copy_kernel(const u32x4 *__restrict__ src, u32x4 *__restrict__ d
 // y[i] = sum over v of ypart[v*stride + i] in part order (the host merge of
 // spmv/2D/DCSR/host/app.c:340-351 and its RBD/BD twins).
 template <typename T>
-__global__ void reduce_parts_kernel(const acc_t<T> *__restrict__ ypart, uint32_t vert, uint64_t stride,
+__global__ void reduce_parts_kernel(const T *__restrict__ ypart, uint32_t vert, uint64_t stride,
                                     T *__restrict__ y, uint64_t n, int k, int ldy) {
     using A = acc_t<T>;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * (uint64_t)k) return;
     const uint64_t row = i / k, q = i % k;
-    A s = ypart[i];
-    for (uint32_t v = 1; v < vert; v++) s = s + ypart[(uint64_t)v * stride + i];
-    y[row * ldy + q] = to_val<T>(s);
+    // val_dt partials added in stripe order, as the reference merge does.
+    T s = ypart[i];
+    for (uint32_t v = 1; v < vert; v++) s = to_val<T>((A)s + (A)ypart[(uint64_t)v * stride + i]);
+    y[row * ldy + q] = s;
 }
 
 }  // namespace spd
@@ -92,10 +93,10 @@ int launch_reduce_parts(hipStream_t s, int dtype, const void *ypart, uint32_t ve
     if (!total) return SP_OK;
     const dim3 grid((unsigned)((total + 255) / 256)), block(256);
     switch (dtype) {
-    case SP_I8: hipLaunchKernelGGL(reduce_parts_kernel<int8_t>, grid, block, 0, s, (const uint32_t *)ypart, vert, stride, (int8_t *)y, n, k, ldy); break;
-    case SP_I16: hipLaunchKernelGGL(reduce_parts_kernel<int16_t>, grid, block, 0, s, (const uint32_t *)ypart, vert, stride, (int16_t *)y, n, k, ldy); break;
-    case SP_I32: hipLaunchKernelGGL(reduce_parts_kernel<int32_t>, grid, block, 0, s, (const uint32_t *)ypart, vert, stride, (int32_t *)y, n, k, ldy); break;
-    case SP_I64: hipLaunchKernelGGL(reduce_parts_kernel<int64_t>, grid, block, 0, s, (const uint64_t *)ypart, vert, stride, (int64_t *)y, n, k, ldy); break;
+    case SP_I8: hipLaunchKernelGGL(reduce_parts_kernel<int8_t>, grid, block, 0, s, (const int8_t *)ypart, vert, stride, (int8_t *)y, n, k, ldy); break;
+    case SP_I16: hipLaunchKernelGGL(reduce_parts_kernel<int16_t>, grid, block, 0, s, (const int16_t *)ypart, vert, stride, (int16_t *)y, n, k, ldy); break;
+    case SP_I32: hipLaunchKernelGGL(reduce_parts_kernel<int32_t>, grid, block, 0, s, (const int32_t *)ypart, vert, stride, (int32_t *)y, n, k, ldy); break;
+    case SP_I64: hipLaunchKernelGGL(reduce_parts_kernel<int64_t>, grid, block, 0, s, (const int64_t *)ypart, vert, stride, (int64_t *)y, n, k, ldy); break;
     case SP_F32: hipLaunchKernelGGL(reduce_parts_kernel<float>, grid, block, 0, s, (const float *)ypart, vert, stride, (float *)y, n, k, ldy); break;
     case SP_F64: hipLaunchKernelGGL(reduce_parts_kernel<double>, grid, block, 0, s, (const double *)ypart, vert, stride, (double *)y, n, k, ldy); break;
     default: return SP_ERR_ARG;

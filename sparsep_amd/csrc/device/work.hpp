@@ -1,0 +1,62 @@
+// Host-side description of one launch group over a list of sub-problems,
+// and the typed runners that launch it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "kern.hpp"
+
+struct sp_ctx;
+
+enum WorkKind { WK_CSR_MERGE = 0, WK_CSR_VEC = 1, WK_COO_SEG = 2, WK_BCSR_VEC = 3, WK_BCOO_SEG = 4 };
+
+struct Work {
+    int kind = WK_CSR_MERGE;
+    int dtype = 0;
+    uint32_t nprob = 0, nunits = 0;
+    spd::Prob *d_prob = nullptr;
+    uint32_t *d_unit_pid = nullptr;
+    uint32_t *d_c0 = nullptr, *d_c1 = nullptr;  // coordinates, index unit + pid
+    uint32_t ipt = 8, lanes = 1, R = 1, C = 1;
+    uint32_t *d_carry_row = nullptr;
+    void *d_carry_val = nullptr;
+    uint32_t carry_k = 0;  // right-hand sides the carry buffer holds
+    uint32_t nzero = 0;
+    uint32_t *d_zpid = nullptr, *d_zrow = nullptr;
+    bool zero_all = false;  // zero every output row first (2D partial buffers)
+    uint64_t zero_rows = 0; // rows to zero when zero_all (times ldy elements)
+};
+
+// Launches `w` for k right-hand sides (processed in chunks of 8/4/2/1).
+// x/y: per-call bases (element offsets from the descriptors apply), or
+// xtab/ytab: per-problem device pointer tables.
+int run_work(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintptr_t *xtab,
+             const uintptr_t *ytab, int k, int ldx, int ldy);
+
+template <typename T>
+int run_work_t(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintptr_t *xtab,
+               const uintptr_t *ytab, int k, int ldx, int ldy);
+
+extern template int run_work_t<int8_t>(const Work &, sp_ctx *, const void *, void *, const uintptr_t *,
+                                       const uintptr_t *, int, int, int);
+extern template int run_work_t<int16_t>(const Work &, sp_ctx *, const void *, void *, const uintptr_t *,
+                                        const uintptr_t *, int, int, int);
+extern template int run_work_t<int32_t>(const Work &, sp_ctx *, const void *, void *, const uintptr_t *,
+                                        const uintptr_t *, int, int, int);
+extern template int run_work_t<int64_t>(const Work &, sp_ctx *, const void *, void *, const uintptr_t *,
+                                        const uintptr_t *, int, int, int);
+extern template int run_work_t<float>(const Work &, sp_ctx *, const void *, void *, const uintptr_t *,
+                                      const uintptr_t *, int, int, int);
+extern template int run_work_t<double>(const Work &, sp_ctx *, const void *, void *, const uintptr_t *,
+                                       const uintptr_t *, int, int, int);
+
+// Host view of one sub-problem while a Work is being built.
+struct HostProb {
+    spd::Prob dev{};               // device pointers + sizes
+    const uint32_t *h_ptr = nullptr;   // CSR rowptr / BCSR browptr (host)
+    const uint32_t *h_rind = nullptr;  // COO rowind / BCOO browind (host)
+    std::vector<uint32_t> unit_split;  // coordinates of this problem's units (size units+1)
+};

@@ -1,38 +1,25 @@
-"""Helpers for the GPU parity tests: device buffers through torch, one SpMV
-through the C ABI, and the parity check against the oracle."""
+"""Helpers for the GPU parity tests: device buffers through the C ABI
+(sp_dmalloc / sp_memcpy_*), one SpMV per call, and the parity check against
+the oracle.  torch is deliberately not imported here: it bundles its own HIP
+runtime, and one process must use a single runtime."""
 import numpy as np
 
 import oracle_lib as orc
 
-TORCH_DT = None
-
-
-def torch_mod():
-    import torch
-    return torch
-
-
-def dev(a):
-    torch = torch_mod()
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
-
 
 def run(sp, ctx, kernel, A, x, o=None, k=1, plan=None):
     """y = A x (or Y = A X for k > 1, X of shape (ncols, k)) through sp_spmv/sp_spmm."""
-    torch = torch_mod()
     plan = plan or sp.Plan(ctx, kernel, A, o)
-    xd = dev(x)
+    xd = sp.DeviceArray.from_numpy(ctx, x)
+    shape = (A.nrows,) if k == 1 else (A.nrows, k)
+    yd = sp.DeviceArray(ctx, shape, x.dtype)
+    yd.fill_bytes(0x5A)  # poison: every row must be written
     if k == 1:
-        yd = torch.full((A.nrows,), 0x5A, dtype=xd.dtype, device="cuda")
+        plan.spmv(xd.addr, yd.addr)
     else:
-        yd = torch.full((A.nrows, k), 0x5A, dtype=xd.dtype, device="cuda")
-    torch.cuda.synchronize()
-    if k == 1:
-        plan.spmv(xd.data_ptr(), yd.data_ptr())
-    else:
-        plan.spmm(k, xd.data_ptr(), k, yd.data_ptr(), k)
+        plan.spmm(k, xd.addr, k, yd.addr, k)
     ctx.sync()
-    return yd.cpu().numpy(), plan
+    return yd.numpy(), plan
 
 
 def oracle_y(dt, A, x):

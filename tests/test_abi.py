@@ -31,8 +31,7 @@ def test_kernel_names(sp):
 
 
 def test_ctx_without_gpu_reports_error(sp):
-    import torch
-    if torch.cuda.is_available():
+    if sp.gpu_present():
         return
     import ctypes
     p = ctypes.c_void_p()

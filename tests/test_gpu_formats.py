@@ -1,0 +1,101 @@
+"""GPU parity of the nine 1D variants (CSR-row .. BCOO-nnz) against the
+oracle, over dtypes, matrix families, the reference partition knobs
+(parts = ex NR_DPUS, sub-parts = ex NR_TASKLETS, tasklet balance) and block
+sizes.  Integers bit-exact; floats within the gamma(len+2)*sum|a x| bound.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import assert_parity, assert_parity_multi, run
+
+pytestmark = pytest.mark.gpu
+
+K1D = ["CSR-row", "CSR-nnz", "COO-row", "COO-nnz-rgrn", "COO-nnz", "BCSR-block", "BCSR-nnz", "BCOO-block",
+       "BCOO-nnz"]
+
+
+def matrices(sp, dt):
+    return {
+        "stencil5": sp.gen_stencil5(48, 40, dt, sp.dist(0, -8, 8), 4.0, seed=1),
+        "rmat": sp.gen_rmat(12, 50000, dtype=dt, seed=5),
+        "uniform": sp.gen_uniform(3000, 4100, 0, 30, dtype=dt, seed=8),
+        "stencil27": sp.gen_stencil27(6, 4, True, dt, seed=3),
+        "banded": sp.gen_banded(2500, 32, 2, 64, dtype=dt, seed=9),
+    }
+
+
+OPTS = [
+    ("default", {}),
+    ("p16s4", dict(nparts=16, nsub=4)),
+    ("p64s16-rowbal", dict(nparts=64, nsub=16, sub_balance=0)),
+    ("p3s1", dict(nparts=3, nsub=1)),
+]
+
+
+@pytest.mark.parametrize("dt", [0, 1, 2, 3, 4, 5])
+def test_all_1d_variants(sp, ctx, dt):
+    for mname, A in matrices(sp, dt).items():
+        x = sp.gen_vector(dt, A.ncols, seed=17)
+        for kern in K1D:
+            for oname, kw in OPTS:
+                y, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+                assert_parity(dt, y, A, x, f"{mname}/{kern}/{oname}")
+
+
+@pytest.mark.parametrize("rc", [(1, 1), (2, 2), (8, 8), (2, 4), (4, 2)])
+@pytest.mark.parametrize("dt", [2, 4, 5])
+def test_block_sizes(sp, ctx, rc, dt):
+    r, c = rc
+    for A in (sp.gen_rmat(11, 20000, dtype=dt, seed=12), sp.gen_stencil27(5, 4, False, dt, seed=4),
+              sp.gen_uniform(1001, 999, 1, 20, dtype=dt, seed=2)):
+        x = sp.gen_vector(dt, A.ncols, seed=3)
+        for kern in ("BCSR-block", "BCSR-nnz", "BCOO-block", "BCOO-nnz"):
+            for kw in ({}, dict(nparts=8, nsub=4), dict(lanes_per_row=1), dict(lanes_per_row=64)):
+                if kern.startswith("BCOO") and "lanes_per_row" in kw:
+                    continue
+                y, _ = run(sp, ctx, kern, A, x, sp.opts(row_block_size=r, col_block_size=c, **kw))
+                assert_parity(dt, y, A, x, f"{kern} {r}x{c} {kw}")
+
+
+@pytest.mark.parametrize("dt", [2, 4])
+@pytest.mark.parametrize("k", [3, 8])
+def test_spmm_all_kinds(sp, ctx, dt, k):
+    A = sp.gen_rmat(11, 25000, dtype=dt, seed=21)
+    X = np.stack([sp.gen_vector(dt, A.ncols, seed=40 + q) for q in range(k)], axis=1)
+    for kern in K1D:
+        Y, _ = run(sp, ctx, kern, A, X, sp.opts(), k=k)
+        assert_parity_multi(dt, Y, A, X, kern)
+
+
+@pytest.mark.parametrize("dt", [1, 4])
+def test_edge_cases_all_kinds(sp, ctx, dt):
+    rng = np.random.default_rng(0)
+    cases = []
+    n = 9000
+    cases.append((10, n, [5] * n, list(range(n))))                       # one long row
+    cases.append((100, 50, [], []))                                       # no entries
+    r = [0] * 5000 + [7] * 3 + [2999] * 6000                              # long rows, gaps
+    c = list(range(5000)) + [1, 2, 3] + list(range(6000))
+    cases.append((3001, 6000, r, c))
+    cases.append((1, 1, [0], [0]))
+    cases.append((7, 13, [6, 6, 6], [0, 5, 12]))                          # rows not multiple of blocks
+    for i, (nr, nc, rr, cc) in enumerate(cases):
+        vals = rng.integers(-4, 5, size=len(rr)).astype(sp.NP_DTYPE[dt])
+        A = sp.Coo.from_arrays(nr, nc, np.asarray(rr), np.asarray(cc), vals, sort=True)
+        x = rng.integers(-4, 5, size=nc).astype(sp.NP_DTYPE[dt])
+        for kern in K1D:
+            for kw in ({}, dict(nparts=4, nsub=3)):
+                y, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+                assert_parity(dt, y, A, x, f"case{i}/{kern}/{kw}")
+
+
+def test_unsorted_coo_input(sp, ctx):
+    """COO plans sort a copy when the input is not in (row, col) order."""
+    rng = np.random.default_rng(3)
+    A0 = sp.gen_rmat(10, 5000, dtype=sp.I32, seed=3)
+    perm = rng.permutation(A0.nnz)
+    A = sp.Coo.from_arrays(A0.nrows, A0.ncols, A0.rowind[perm], A0.colind[perm], A0.values[perm])
+    x = sp.gen_vector(sp.I32, A.ncols, seed=4)
+    for kern in K1D:
+        y, _ = run(sp, ctx, kern, A, x)
+        assert_parity(2, y, A, x, kern)
