@@ -360,10 +360,16 @@ int sp_partition_tsklt_bcoo(const uint32_t *nnz_per_block, const uint32_t *block
 /* find_partitions (spmv/2D/DCSR/host/app.c:66-69): horz = n / vert. */
 uint32_t sp_find_partitions(uint32_t nparts, uint32_t vert);
 
-/* RBD/BD per-stripe row splits at row granularity
- * (spmv/2D/RBDCSR/support/partition.h:45-89, BDCSR/support/partition.h:47-91):
- * row_split[vert*(horz+1)]. */
-int sp_partition_tiled_rows(const sp_tiled *A, uint32_t horz, uint32_t *row_split);
+/* RBD/BD per-stripe splits into `horz` parts, split[c*(horz+1) + i],
+ * stripe-local: row splits (CSR; spmv/2D/RBDCSR/support/partition.h:45-89),
+ * block-row splits (BCSR; by blocks spmv/2D/RBDBCSR-block/support/partition.h:50,
+ * by non-zeros spmv/2D/RBDBCSR-nnz/support/partition.h:51), entry splits
+ * (COO; nnz_split of spmv/2D/RBDCOO/support/partition.h:45-118) or block
+ * splits (BCOO; spmv/2D/RBDBCOO-block/support/partition.h:50,
+ * spmv/2D/RBDBCOO-nnz/support/partition.h:50).  by_nnz selects the
+ * non-zero-balanced rule for the block formats.  SP_ERR_ARG for D tilings
+ * (each D tile is its own part). */
+int sp_partition_tiled(const sp_tiled *A, uint32_t horz, int by_nnz, uint32_t *split);
 
 /* ------------------------------------------------------------------------ */
 /* Device path                                                              */

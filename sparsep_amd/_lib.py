@@ -132,7 +132,7 @@ SIGNATURES = {
     "sp_partition_bcoo": (I, [u32p, U64, U, U, I, u32p, u32p]),
     "sp_partition_tsklt_bcoo": (I, [u32p, u32p, u32p, U, U, I, u32p]),
     "sp_find_partitions": (U, [U, U]),
-    "sp_partition_tiled_rows": (I, [ctypes.POINTER(sp_tiled), U, u32p]),
+    "sp_partition_tiled": (I, [ctypes.POINTER(sp_tiled), U, I, u32p]),
     "sp_kernel_name": (ctypes.c_char_p, [I]),
     "sp_opts_default": (V, [ctypes.POINTER(sp_opts)]),
     "sp_ctx_create": (I, [I, PP]),

@@ -344,9 +344,9 @@ def partition_tsklt_bcoo(npb, block_split, nnzs, part, ntasklets, by_nnz):
     return out
 
 
-def partition_tiled_rows(T, horz):
+def partition_tiled(T, horz, by_nnz=False):
     out = np.zeros(T.vert_partitions * (horz + 1), np.uint32)
-    check(lib().sp_partition_tiled_rows(T.ptr, horz, out.ctypes.data_as(_lib.u32p)))
+    check(lib().sp_partition_tiled(T.ptr, horz, int(by_nnz), out.ctypes.data_as(_lib.u32p)))
     return out
 
 

@@ -456,3 +456,254 @@ int run_work(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintptr
     default: return SP_ERR_ARG;
     }
 }
+
+// ---------------------------------------------------------------------------
+// 2D tilings (ids 9..24): one sub-problem per tile, each writing its
+// stripe's partial rows into ypart[stripe][row]; the reduction then adds
+// stripes in order (the reference host merge, spmv/2D/DCSR/host/app.c:340-351).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TiledKind {
+    int tiling, format, by_nnz;
+};
+
+TiledKind tiled_kind(int id) {
+    switch (id) {
+    case SP_K_DCSR: return {SP_TILE_D, SP_FMT_CSR, 0};
+    case SP_K_DCOO: return {SP_TILE_D, SP_FMT_COO, 0};
+    case SP_K_DBCSR: return {SP_TILE_D, SP_FMT_BCSR, 0};
+    case SP_K_DBCOO: return {SP_TILE_D, SP_FMT_BCOO, 0};
+    case SP_K_RBDCSR: return {SP_TILE_RB, SP_FMT_CSR, 0};
+    case SP_K_RBDCOO: return {SP_TILE_RB, SP_FMT_COO, 0};
+    case SP_K_RBDBCSR_BLOCK: return {SP_TILE_RB, SP_FMT_BCSR, 0};
+    case SP_K_RBDBCSR_NNZ: return {SP_TILE_RB, SP_FMT_BCSR, 1};
+    case SP_K_RBDBCOO_BLOCK: return {SP_TILE_RB, SP_FMT_BCOO, 0};
+    case SP_K_RBDBCOO_NNZ: return {SP_TILE_RB, SP_FMT_BCOO, 1};
+    case SP_K_BDCSR: return {SP_TILE_BD, SP_FMT_CSR, 0};
+    case SP_K_BDCOO: return {SP_TILE_BD, SP_FMT_COO, 0};
+    case SP_K_BDBCSR_BLOCK: return {SP_TILE_BD, SP_FMT_BCSR, 0};
+    case SP_K_BDBCSR_NNZ: return {SP_TILE_BD, SP_FMT_BCSR, 1};
+    case SP_K_BDBCOO_BLOCK: return {SP_TILE_BD, SP_FMT_BCOO, 0};
+    default: return {SP_TILE_BD, SP_FMT_BCOO, 1};
+    }
+}
+
+}  // namespace
+
+int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
+    const TiledKind tk = tiled_kind(p->id);
+    const uint32_t n = A0->nrows;
+    const size_t sv = sp_dtype_size(A0->dtype);
+    uint32_t vert = p->opts.vert ? p->opts.vert : 2;
+    if (vert > A0->ncols && A0->ncols) vert = A0->ncols;
+    if (vert == 0) vert = 1;
+    // horz: D tiles per column stripe (ex NR_DPUS / vert); RB/BD row parts per stripe.
+    uint32_t horz;
+    if (tk.tiling == SP_TILE_D) {
+        horz = p->opts.nparts ? std::max<uint32_t>(1, sp_find_partitions(p->opts.nparts, vert))
+                              : std::max<uint32_t>(1, (n + 8191) / 8192);
+        if (horz > n && n) horz = n;
+    } else {
+        horz = p->opts.nparts ? std::max<uint32_t>(1, sp_find_partitions(p->opts.nparts, vert)) : 1;
+    }
+    // tilings need the sorted COO
+    sp_coo *sorted = nullptr;
+    const sp_coo *A = A0;
+    if (!coo_sorted(A0)) {
+        int rc = sp_coo_alloc(A0->nrows, A0->ncols, A0->nnz, A0->dtype, &sorted);
+        if (rc) return rc;
+        memcpy(sorted->rowind, A0->rowind, (size_t)A0->nnz * 4);
+        memcpy(sorted->colind, A0->colind, (size_t)A0->nnz * 4);
+        memcpy(sorted->values, A0->values, (size_t)A0->nnz * sv);
+        if ((rc = sp_sort_coo(sorted))) { sp_free_coo(sorted); return rc; }
+        A = sorted;
+    }
+    sp_tiled *T = nullptr;
+    int rc = sp_coo2tiled(A, tk.tiling, tk.format, horz, vert, p->opts.row_block_size, p->opts.col_block_size,
+                          p->opts.count_values, &T);
+    // row statistics for the CSR algorithm choice
+    uint32_t maxlen = 0;
+    if (!rc && tk.format == SP_FMT_CSR && p->algo == SP_ALGO_AUTO) {
+        std::vector<uint32_t> cnt(n + 1, 0);
+        for (uint32_t k = 0; k < A->nnz; k++) cnt[A->rowind[k]]++;
+        for (uint32_t i = 0; i < n; i++) maxlen = std::max(maxlen, cnt[i]);
+    }
+    if (sorted) sp_free_coo(sorted);
+    if (rc) return rc;
+
+    p->tiled = true;
+    p->vert = vert;
+    p->ntiles = T->ntiles;
+    p->nparts = tk.tiling == SP_TILE_D ? T->ntiles : vert * horz;
+    const uint32_t R = p->opts.row_block_size, C = p->opts.col_block_size;
+    const uint32_t nt = T->ntiles, th = T->tile_height;
+    Builder b(p);
+    void *d;
+    do {
+        // upload the tiling's arrays once; tiles point into them
+        const uint32_t *d_ptr = nullptr, *d_rind = nullptr, *d_cind = nullptr;
+        const char *d_val = nullptr;
+        const bool blk = tk.format >= SP_FMT_BCSR;
+        const uint64_t nent = blk ? T->num_blocks : T->nnz;
+        const size_t esz = blk ? (size_t)R * C * sv : sv;
+        if (tk.format == SP_FMT_CSR) {
+            if ((rc = plan_upload(p, T->drowptr, (size_t)nt * (th + 1) * 4, &d))) break;
+            d_ptr = (const uint32_t *)d;
+        } else if (tk.format == SP_FMT_BCSR) {
+            if ((rc = plan_upload(p, T->browptr, (size_t)nt * (T->num_block_rows + 1) * 4, &d))) break;
+            d_ptr = (const uint32_t *)d;
+        }
+        if (tk.format == SP_FMT_COO || tk.format == SP_FMT_BCOO) {
+            if ((rc = plan_upload(p, tk.format == SP_FMT_COO ? T->drowind : T->browind, nent * 4, &d))) break;
+            d_rind = (const uint32_t *)d;
+        }
+        if ((rc = plan_upload(p, blk ? T->bcolind : T->dcolind, nent * 4, &d))) break;
+        d_cind = (const uint32_t *)d;
+        if ((rc = plan_upload(p, blk ? T->bval : T->dval, nent * esz, &d))) break;
+        d_val = (const char *)d;
+
+        std::vector<uint32_t> split;
+        if (tk.tiling != SP_TILE_D) {
+            split.resize((size_t)vert * (horz + 1));
+            if ((rc = sp_partition_tiled(T, horz, tk.by_nnz, split.data()))) break;
+        }
+        int kind;
+        if (tk.format == SP_FMT_CSR) {
+            if (p->algo == SP_ALGO_AUTO) {
+                const double avg = n ? (double)A0->nnz / n : 0.0;
+                p->algo = (maxlen <= 4 * avg + 64) ? SP_ALGO_VECTOR : SP_ALGO_SEGMENTED;
+            }
+            kind = p->algo == SP_ALGO_VECTOR ? WK_CSR_VEC : WK_CSR_MERGE;
+        } else if (tk.format == SP_FMT_COO) {
+            kind = WK_COO_SEG;
+            p->algo = SP_ALGO_SEGMENTED;
+        } else if (tk.format == SP_FMT_BCSR) {
+            kind = WK_BCSR_VEC;
+            p->algo = SP_ALGO_VECTOR;
+        } else {
+            kind = WK_BCOO_SEG;
+            p->algo = SP_ALGO_SEGMENTED;
+        }
+        const uint32_t I = pick_ipt(p->opts, 8);
+        if (!I) { rc = SP_ERR_ARG; break; }
+        uint32_t L = p->opts.lanes_per_row;
+        if (kind == WK_CSR_VEC && L == 0) L = auto_lanes(n ? (double)A0->nnz / n / vert : 1.0);
+        if (kind == WK_BCSR_VEC && L == 0) {
+            const double avg = T->num_block_rows ? (double)T->num_blocks / ((double)T->num_block_rows * nt) : 1.0;
+            L = avg >= 48 ? 64 : avg >= 12 ? 16 : avg >= 3 ? 4 : 1;
+        }
+        p->lanes = p->work.lanes = L;
+        p->ipt = p->work.ipt = kind == WK_BCOO_SEG ? 4 : I;
+        p->work.R = R;
+        p->work.C = C;
+        const uint32_t tile_items = TPB * p->work.ipt;
+        for (uint32_t t = 0; t < nt && !rc; t++) {
+            const uint32_t c = t % vert;
+            const uint32_t row_off = tk.tiling == SP_TILE_D ? (t / vert) * th : 0;
+            const uint32_t rows = row_off < n ? std::min(th, n - row_off) : 0;
+            Prob P{};
+            P.nrows_elem = rows;
+            P.ncols_elem = T->vert_tile_widths[c + 1] - T->vert_tile_widths[c];
+            P.xoff = T->vert_tile_widths[c];
+            P.yoff = (int64_t)c * n + row_off;
+            std::vector<uint32_t> k0, k1;
+            if (!blk) {
+                const uint64_t e0 = T->nnz_offset[t];
+                P.nent = T->nnzs_per_partition[t];
+                P.cind = d_cind + e0;
+                P.val = d_val + e0 * sv;
+                P.nrows = rows;
+                if (tk.format == SP_FMT_CSR) {
+                    P.ptr = d_ptr + (size_t)t * (th + 1);
+                    const uint32_t *hrp = T->drowptr + (size_t)t * (th + 1);
+                    if (rows == 0) {
+                    } else if (kind == WK_CSR_MERGE) {
+                        merge_coords(hrp, rows, hrp[rows], tile_items, k0, k1);
+                    } else {
+                        const uint32_t gran = (uint32_t)(8 / sv);
+                        const uint64_t rpu = 4ULL * (TPB / L);
+                        push_cut(k0, 0);
+                        const uint32_t parts = tk.tiling == SP_TILE_D ? 1 : horz;
+                        for (uint32_t q = 0; q < parts && !rc; q++) {
+                            const uint32_t r0 = tk.tiling == SP_TILE_D ? 0 : std::min(split[(size_t)c * (horz + 1) + q], rows);
+                            const uint32_t r1 = tk.tiling == SP_TILE_D ? rows : std::min(std::max(split[(size_t)c * (horz + 1) + q + 1], r0), rows);
+                            if (r1 == r0) continue;
+                            const uint32_t Tn = p->opts.nsub ? p->opts.nsub
+                                                             : (uint32_t)std::max<uint64_t>(1, (r1 - r0 + rpu - 1) / rpu);
+                            rc = row_units(hrp, r0, r1, Tn, p->opts.sub_balance == SP_BAL_NNZ, gran, k0);
+                        }
+                        push_cut(k0, rows);
+                    }
+                } else {  // COO tile
+                    P.rind = d_rind + e0;
+                    const uint32_t cnt = P.nent;
+                    if (cnt) {
+                        if (tk.tiling == SP_TILE_D) {
+                            tile_range(0, cnt, tile_items, k0);
+                        } else {
+                            push_cut(k0, 0);
+                            for (uint32_t q = 0; q < horz; q++) {
+                                const uint32_t s0 = std::min(split[(size_t)c * (horz + 1) + q], cnt);
+                                const uint32_t s1 = std::min(std::max(split[(size_t)c * (horz + 1) + q + 1], s0), cnt);
+                                if (s1 > s0) tile_range(s0, s1, tile_items, k0);
+                            }
+                            push_cut(k0, cnt);
+                        }
+                    }
+                }
+            } else {
+                const uint64_t b0 = T->block_offset[t];
+                const uint32_t nbr_t = (rows + R - 1) / R;
+                P.nent = T->blocks_per_partition[t];
+                P.cind = d_cind + b0;
+                P.val = d_val + b0 * esz;
+                P.nrows = nbr_t;
+                if (tk.format == SP_FMT_BCSR) {
+                    P.ptr = d_ptr + (size_t)t * (T->num_block_rows + 1);
+                    const uint32_t *hbp = T->browptr + (size_t)t * (T->num_block_rows + 1);
+                    if (nbr_t) {
+                        const uint64_t bpu = 4ULL * (TPB / L);
+                        push_cut(k0, 0);
+                        const uint32_t parts = tk.tiling == SP_TILE_D ? 1 : horz;
+                        for (uint32_t q = 0; q < parts; q++) {
+                            const uint32_t r0 = tk.tiling == SP_TILE_D ? 0 : std::min(split[(size_t)c * (horz + 1) + q], nbr_t);
+                            const uint32_t r1 = tk.tiling == SP_TILE_D ? nbr_t : std::min(std::max(split[(size_t)c * (horz + 1) + q + 1], r0), nbr_t);
+                            if (r1 == r0) continue;
+                            const uint32_t Tn = p->opts.nsub ? p->opts.nsub
+                                                             : (uint32_t)std::max<uint64_t>(1, (r1 - r0 + bpu - 1) / bpu);
+                            // even block-row sub-parts (the tasklet split is a work assignment only)
+                            for (uint32_t s = 0; s <= Tn; s++) push_cut(k0, r0 + (uint32_t)((uint64_t)(r1 - r0) * s / Tn));
+                        }
+                        push_cut(k0, nbr_t);
+                        (void)hbp;
+                    }
+                } else {  // BCOO tile
+                    P.rind = d_rind + b0;
+                    const uint32_t cnt = P.nent;
+                    if (cnt) {
+                        if (tk.tiling == SP_TILE_D) {
+                            tile_range(0, cnt, tile_items, k0);
+                        } else {
+                            push_cut(k0, 0);
+                            for (uint32_t q = 0; q < horz; q++) {
+                                const uint32_t s0 = std::min(split[(size_t)c * (horz + 1) + q], cnt);
+                                const uint32_t s1 = std::min(std::max(split[(size_t)c * (horz + 1) + q + 1], s0), cnt);
+                                if (s1 > s0) tile_range(s0, s1, tile_items, k0);
+                            }
+                            push_cut(k0, cnt);
+                        }
+                    }
+                }
+            }
+            b.add(P, k0, kind == WK_CSR_MERGE ? &k1 : nullptr);
+        }
+        if (rc) break;
+        const bool carries = kind == WK_CSR_MERGE || kind == WK_COO_SEG || kind == WK_BCOO_SEG;
+        rc = b.finish(kind, carries, kind == WK_BCOO_SEG ? R : 1);
+        p->work.zero_all = (kind == WK_COO_SEG || kind == WK_BCOO_SEG);
+        p->nsub = p->work.nunits;
+    } while (0);
+    sp_free_tiled(T);
+    return rc;
+}

@@ -1,0 +1,64 @@
+"""GPU parity of the sixteen 2D variants (DCSR .. BDBCOO-nnz): tiles write
+per-stripe partial rows that a reduction adds in stripe order.  Integers
+bit-exact; floats within the gamma(len+2)*sum|a x| bound."""
+import numpy as np
+import pytest
+
+from gpu_util import assert_parity, assert_parity_multi, run
+
+pytestmark = pytest.mark.gpu
+
+K2D = ["DCSR", "DCOO", "DBCSR", "DBCOO", "RBDCSR", "RBDCOO", "RBDBCSR-block", "RBDBCSR-nnz", "RBDBCOO-block",
+       "RBDBCOO-nnz", "BDCSR", "BDCOO", "BDBCSR-block", "BDBCSR-nnz", "BDBCOO-block", "BDBCOO-nnz"]
+
+
+@pytest.mark.parametrize("dt", [0, 2, 4, 5])
+def test_all_2d_variants(sp, ctx, dt):
+    mats = {
+        "rmat": sp.gen_rmat(12, 40000, dtype=dt, seed=5),
+        "stencil27": sp.gen_stencil27(6, 4, True, dt, seed=3),
+        "uniform": sp.gen_uniform(2000, 3001, 0, 25, dtype=dt, seed=8),
+    }
+    for mname, A in mats.items():
+        x = sp.gen_vector(dt, A.ncols, seed=17)
+        for kern in K2D:
+            for kw in (dict(vert=1), dict(vert=2), dict(vert=3, nparts=24), dict(vert=8, nparts=64, nsub=4)):
+                y, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+                assert_parity(dt, y, A, x, f"{mname}/{kern}/{kw}")
+
+
+@pytest.mark.parametrize("dt", [2, 4])
+def test_2d_csr_algorithms_and_blocks(sp, ctx, dt):
+    A = sp.gen_rmat(11, 25000, dtype=dt, seed=9)
+    x = sp.gen_vector(dt, A.ncols, seed=2)
+    for kern in ("DCSR", "RBDCSR", "BDCSR"):
+        for kw in (dict(algo=1, lanes_per_row=4), dict(algo=2, items_per_lane=4), dict(algo=2, items_per_lane=16)):
+            y, _ = run(sp, ctx, kern, A, x, sp.opts(vert=4, **kw))
+            assert_parity(dt, y, A, x, f"{kern}/{kw}")
+    for kern in ("DBCSR", "RBDBCSR-nnz", "BDBCOO-block"):
+        for r, c in ((2, 2), (8, 8), (2, 4)):
+            y, _ = run(sp, ctx, kern, A, x, sp.opts(vert=3, row_block_size=r, col_block_size=c))
+            assert_parity(dt, y, A, x, f"{kern}/{r}x{c}")
+
+
+@pytest.mark.parametrize("k", [2, 5])
+def test_2d_spmm(sp, ctx, k):
+    A = sp.gen_rmat(10, 9000, dtype=4, seed=4)
+    X = np.stack([sp.gen_vector(4, A.ncols, seed=60 + q) for q in range(k)], axis=1)
+    for kern in K2D:
+        Y, _ = run(sp, ctx, kern, A, X, sp.opts(vert=3), k=k)
+        assert_parity_multi(4, Y, A, X, kern)
+
+
+def test_2d_edge_cases(sp, ctx):
+    rng = np.random.default_rng(1)
+    # more stripes than columns with entries, tiles past the last row, empty matrix
+    cases = [(37, 5, [0, 3, 36, 36], [4, 0, 1, 4]), (10, 100, [], []), (1000, 7, list(range(0, 1000, 3)), [6] * 334)]
+    for nr, nc, r, c in cases:
+        v = rng.integers(-4, 5, len(r)).astype(np.int32)
+        A = sp.Coo.from_arrays(nr, nc, np.asarray(r), np.asarray(c), v, sort=True)
+        x = rng.integers(-4, 5, nc).astype(np.int32)
+        for kern in K2D:
+            for kw in (dict(vert=4), dict(vert=16, nparts=64)):
+                y, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+                assert_parity(2, y, A, x, f"{nr}x{nc}/{kern}/{kw}")
