@@ -483,9 +483,89 @@ int sp_time_spmm(sp_plan *plan, int k, const void *X, int ldx, void *Y, int ldy,
 int sp_time_spmv_split(sp_plan *plan, const void *x, void *y, int iters, uint64_t flush_bytes,
                        float *ms, float *ms_dominant);
 
+/* Batched SpMV / SpMM (no counterpart in the reference, which runs one
+ * matrix per launch): the problems of `nprob` 1D plans (same dtype) with
+ * k[i] right-hand sides each run as persistent launches, one per
+ * (kernel kind, tile shape, RHS width) class, sweeping every problem's work
+ * units; with SP_BATCH_GRAPH they are captured once into a hipGraph and
+ * replayed.  X[i] is ncols_i x k[i] row-major (leading dimension k[i]),
+ * Y[i] nrows_i x k[i]; both device pointers, given per call.  A batch
+ * keeps device state: calls on one batch must not overlap. */
+typedef struct sp_batch sp_batch;
+enum { SP_BATCH_GRAPH = 1 };
+int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, const int *k, int flags,
+                    sp_batch **out);
+int sp_batch_run(sp_batch *b, const void *const *X, void *const *Y);
+int sp_batch_time(sp_batch *b, const void *const *X, void *const *Y, int iters,
+                  uint64_t flush_bytes, float *ms);
+int sp_batch_info(const sp_batch *b, uint32_t *ngroups, uint64_t *algo_bytes);
+int sp_batch_destroy(sp_batch *b);
+/* create + run + destroy */
+int sp_batch_run_once(sp_ctx *ctx, int nprob, sp_plan *const *plans, const void *const *X,
+                      void *const *Y, const int *k);
+
+/* Exhaustive autotuner (no counterpart in the reference).  A point is one
+ * kernel id with one option set; sp_tune_space enumerates the candidate
+ * points for matrix A (kernel_id < 0: all 25 ids), sp_tune_run builds and
+ * times each point (median of `iters` SpMVs with HIP events, L2/MALL
+ * flushed with flush_bytes between runs) and fills ms / build_ms / status
+ * (0 ok, else the sp_status of plan creation), sp_tune_best returns the
+ * index of the fastest point (of kernel_id, or of all when < 0), and
+ * sp_tune_db_append appends the points as JSON lines to a timing database. */
+typedef struct sp_tune_point {
+    int32_t kernel_id;
+    sp_opts opts;
+    float ms;          /* median SpMV time, -1 when not measured */
+    float build_ms;    /* plan creation time */
+    int32_t status;
+    uint64_t algo_bytes;
+} sp_tune_point;
+int sp_tune_space(const sp_coo *A, int kernel_id, sp_tune_point *pts, int cap, int *count);
+int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters,
+                uint64_t flush_bytes);
+int sp_tune_best(const sp_tune_point *pts, int n, int kernel_id);
+int sp_tune_db_append(const char *path, const char *tag, const sp_tune_point *pts, int n);
+
 /* Bandwidth probe: time a device copy of `bytes` (read + write), returns
  * ms per copy; the measured stream-copy peak is 2*bytes/ms. */
 int sp_time_copy(sp_ctx *ctx, uint64_t bytes, int iters, float *ms);
+
+/* ---------------------------------------------------------------------
+ * Learned variant selector (no counterpart in the reference; its only
+ * precursor is the block fill ratio computed in
+ * spmv/1D/BCOO-block/support/matrix.h:491-505).
+ * ------------------------------------------------------------------- */
+
+/* Cheap structural features of A (SP_NFEAT floats, all O(nnz) host work):
+ * size logs, row-length moments and log2 histogram, column-length moments,
+ * bandwidth and diagonal-distance histogram, skew (Gini, max/avg), 2x2 and
+ * 4x4 block fill ratios, gather locality, dtype one-hot.  A may be unsorted. */
+enum { SP_NFEAT = 64 };
+int sp_features(const sp_coo *A, float *feat);
+
+/* MLP in -> h1 -> h2 -> out with ReLU, BF16 weights (rounded copies of FP32
+ * master weights) and FP32 activations / accumulation, trained with Adam by
+ * hand-written HIP kernels.  Parameter layout (FP32, flat):
+ * W1[h1][in] b1[h1] W2[h2][h1] b2[h2] W3[out][h2] b3[out].
+ * Loss = mean over all B*out outputs of (M*(Y-T))^2 (M = 1 when NULL).
+ * X, T, M, Y are host arrays, row-major [B][in] / [B][out]. */
+typedef struct sp_mlp sp_mlp;
+int sp_mlp_create(sp_ctx *ctx, int in, int h1, int h2, int out, uint64_t seed, sp_mlp **m);
+int sp_mlp_destroy(sp_mlp *m);
+size_t sp_mlp_num_params(const sp_mlp *m);
+int sp_mlp_set_params(sp_mlp *m, const float *params, size_t n);  /* resets Adam step count */
+int sp_mlp_get_params(sp_mlp *m, float *params, size_t n);
+int sp_mlp_predict(sp_mlp *m, const float *X, int B, float *Y);
+/* Forward + backward of one batch: loss (may be NULL) and the gradient with
+ * respect to the FP32 parameters (grad may be NULL; kept on the device for
+ * sp_mlp_adam either way). */
+int sp_mlp_loss_grad(sp_mlp *m, const float *X, const float *T, const float *M, int B,
+                     float *loss, float *grad);
+int sp_mlp_adam(sp_mlp *m, float lr, float beta1, float beta2, float eps);
+/* Shuffled mini-batch Adam (beta 0.9/0.999, eps 1e-8) over n samples;
+ * final_loss = mean batch loss of the last epoch. */
+int sp_mlp_train(sp_mlp *m, const float *X, const float *T, const float *M, int n, int epochs,
+                 int batch, float lr, uint64_t seed, float *final_loss);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libsparsep.so")
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
 u64p = ctypes.POINTER(ctypes.c_uint64)
+fp = ctypes.POINTER(ctypes.c_float)
 
 
 class sp_coo(ctypes.Structure):
@@ -70,6 +71,11 @@ class sp_opts(ctypes.Structure):
                 ("vert", ctypes.c_uint32), ("row_block_size", ctypes.c_uint32),
                 ("col_block_size", ctypes.c_uint32), ("lds_slice", ctypes.c_uint32),
                 ("count_values", ctypes.c_int32), ("mem_cap_bytes", ctypes.c_uint64)]
+
+
+class sp_tune_point(ctypes.Structure):
+    _fields_ = [("kernel_id", ctypes.c_int32), ("opts", sp_opts), ("ms", ctypes.c_float),
+                ("build_ms", ctypes.c_float), ("status", ctypes.c_int32), ("algo_bytes", ctypes.c_uint64)]
 
 
 class sp_plan_info(ctypes.Structure):
@@ -157,6 +163,26 @@ SIGNATURES = {
     "sp_time_spmv_split": (I, [P, P, P, I, U64, ctypes.POINTER(ctypes.c_float),
                                ctypes.POINTER(ctypes.c_float)]),
     "sp_time_copy": (I, [P, U64, I, ctypes.POINTER(ctypes.c_float)]),
+    "sp_tune_space": (I, [ctypes.POINTER(sp_coo), I, ctypes.POINTER(sp_tune_point), I, ctypes.POINTER(ctypes.c_int)]),
+    "sp_tune_run": (I, [P, ctypes.POINTER(sp_coo), ctypes.POINTER(sp_tune_point), I, I, U64]),
+    "sp_tune_best": (I, [ctypes.POINTER(sp_tune_point), I, I]),
+    "sp_tune_db_append": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(sp_tune_point), I]),
+    "sp_batch_create": (I, [P, I, PP, ctypes.POINTER(ctypes.c_int), I, PP]),
+    "sp_batch_run": (I, [P, PP, PP]),
+    "sp_batch_time": (I, [P, PP, PP, I, U64, ctypes.POINTER(ctypes.c_float)]),
+    "sp_batch_info": (I, [P, ctypes.POINTER(ctypes.c_uint32), u64p]),
+    "sp_batch_destroy": (I, [P]),
+    "sp_batch_run_once": (I, [P, I, PP, PP, PP, ctypes.POINTER(ctypes.c_int)]),
+    "sp_features": (I, [ctypes.POINTER(sp_coo), fp]),
+    "sp_mlp_create": (I, [P, I, I, I, I, U64, PP]),
+    "sp_mlp_destroy": (I, [P]),
+    "sp_mlp_num_params": (S, [P]),
+    "sp_mlp_set_params": (I, [P, fp, S]),
+    "sp_mlp_get_params": (I, [P, fp, S]),
+    "sp_mlp_predict": (I, [P, fp, I, fp]),
+    "sp_mlp_loss_grad": (I, [P, fp, fp, fp, I, fp, fp]),
+    "sp_mlp_adam": (I, [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "sp_mlp_train": (I, [P, fp, fp, fp, I, I, I, ctypes.c_float, U64, fp]),
 }
 
 

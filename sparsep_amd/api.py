@@ -492,3 +492,149 @@ class Plan:
         check(lib().sp_time_spmm(self.ptr, k, ctypes.c_void_p(X_ptr), ldx, ctypes.c_void_p(Y_ptr), ldy,
                                  iters, flush_bytes, ms), "sp_time_spmm")
         return list(ms)
+
+
+class Batch:
+    """Batched SpMV/SpMM over several plans (sp_batch_create)."""
+
+    GRAPH = 1
+
+    def __init__(self, ctx, plans, k, graph=True):
+        self.ctx = ctx
+        self.plans = list(plans)
+        n = len(self.plans)
+        arr = (ctypes.c_void_p * n)(*[p.ptr.value for p in self.plans])
+        ks = (ctypes.c_int * n)(*[int(v) for v in k])
+        self.ptr = ctypes.c_void_p()
+        check(lib().sp_batch_create(ctx.ptr, n, arr, ks, self.GRAPH if graph else 0, ctypes.byref(self.ptr)),
+              "sp_batch_create")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().sp_batch_destroy(self.ptr)
+            self.ptr = None
+
+    @staticmethod
+    def _tab(ptrs):
+        return (ctypes.c_void_p * len(ptrs))(*ptrs)
+
+    def run(self, xs, ys):
+        check(lib().sp_batch_run(self.ptr, self._tab(xs), self._tab(ys)), "sp_batch_run")
+
+    def time(self, xs, ys, iters, flush_bytes=0):
+        ms = (ctypes.c_float * iters)()
+        check(lib().sp_batch_time(self.ptr, self._tab(xs), self._tab(ys), iters, flush_bytes, ms), "sp_batch_time")
+        return list(ms)
+
+    def info(self):
+        ng, ab = ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().sp_batch_info(self.ptr, ctypes.byref(ng), ctypes.byref(ab)))
+        return {"groups": ng.value, "algo_bytes": ab.value}
+
+
+def tune_space(A, kernel_id=-1):
+    """Candidate (kernel id, options) points of the exhaustive autotuner."""
+    n = ctypes.c_int()
+    check(lib().sp_tune_space(A.ptr, kernel_id, None, 0, ctypes.byref(n)))
+    pts = (_lib.sp_tune_point * n.value)()
+    check(lib().sp_tune_space(A.ptr, kernel_id, pts, n.value, ctypes.byref(n)))
+    return pts
+
+
+def tune(ctx, A, pts=None, iters=5, flush_bytes=0):
+    """Build and time every point; returns the point array (ms, status filled)."""
+    pts = pts if pts is not None else tune_space(A)
+    check(lib().sp_tune_run(ctx.ptr, A.ptr, pts, len(pts), iters, flush_bytes), "sp_tune_run")
+    return pts
+
+
+def tune_best(pts, kernel_id=-1):
+    return lib().sp_tune_best(pts, len(pts), kernel_id)
+
+
+def point_opts(pt):
+    """sp_opts of a tune point (a copy)."""
+    o = sp_opts()
+    ctypes.memmove(ctypes.byref(o), ctypes.byref(pt.opts), ctypes.sizeof(sp_opts))
+    return o
+
+
+# --------------------------------------------------------------------------
+# learned variant selector: features and the MLP
+# --------------------------------------------------------------------------
+NFEAT = 64
+
+
+def features(A):
+    """64-float structural feature vector of a Coo (sp_features)."""
+    f = np.zeros(NFEAT, np.float32)
+    check(lib().sp_features(A.ptr, f.ctypes.data_as(_lib.fp)), "sp_features")
+    return f
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(_lib.fp)
+
+
+def _f32(a, shape=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Mlp:
+    """in -> h1 -> h2 -> out MLP trained and run by the library's HIP kernels
+    (BF16 weights, FP32 activations and accumulation)."""
+
+    def __init__(self, ctx, sizes=(64, 128, 128, 25), seed=5):
+        self.ctx = ctx
+        self.sizes = tuple(int(s) for s in sizes)
+        self.ptr = ctypes.c_void_p()
+        check(lib().sp_mlp_create(ctx.ptr, *self.sizes, seed, ctypes.byref(self.ptr)), "sp_mlp_create")
+        self.nparams = lib().sp_mlp_num_params(self.ptr)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib().sp_mlp_destroy(self.ptr)
+            self.ptr = None
+
+    def get_params(self):
+        p = np.empty(self.nparams, np.float32)
+        check(lib().sp_mlp_get_params(self.ptr, _fptr(p), self.nparams))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.nparams
+        check(lib().sp_mlp_set_params(self.ptr, _fptr(p), self.nparams))
+
+    def predict(self, X):
+        X = _f32(X, (-1, self.sizes[0]))
+        Y = np.empty((X.shape[0], self.sizes[-1]), np.float32)
+        check(lib().sp_mlp_predict(self.ptr, _fptr(X), X.shape[0], _fptr(Y)), "sp_mlp_predict")
+        return Y
+
+    def loss_grad(self, X, T, M=None):
+        X = _f32(X, (-1, self.sizes[0]))
+        T = _f32(T, (X.shape[0], self.sizes[-1]))
+        M = _f32(M, (X.shape[0], self.sizes[-1]))
+        loss = ctypes.c_float()
+        g = np.empty(self.nparams, np.float32)
+        check(lib().sp_mlp_loss_grad(self.ptr, _fptr(X), _fptr(T), _fptr(M), X.shape[0],
+                                     ctypes.byref(loss), _fptr(g)), "sp_mlp_loss_grad")
+        return loss.value, g
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        check(lib().sp_mlp_adam(self.ptr, lr, b1, b2, eps))
+
+    def train(self, X, T, M=None, epochs=100, batch=64, lr=1e-3, seed=5):
+        X = _f32(X, (-1, self.sizes[0]))
+        T = _f32(T, (X.shape[0], self.sizes[-1]))
+        M = _f32(M, (X.shape[0], self.sizes[-1]))
+        loss = ctypes.c_float()
+        check(lib().sp_mlp_train(self.ptr, _fptr(X), _fptr(T), _fptr(M), X.shape[0], epochs, batch,
+                                 lr, seed, ctypes.byref(loss)), "sp_mlp_train")
+        return loss.value

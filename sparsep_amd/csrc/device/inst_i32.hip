@@ -4,3 +4,4 @@
 
 template int run_work_t<int32_t>(const Work &, sp_ctx *, const void *, void *, const uintptr_t *, const uintptr_t *,
                             int, int, int);
+template int run_tables_t<int32_t>(const Work &, sp_ctx *, const uintptr_t *, const uintptr_t *, int);

@@ -41,6 +41,7 @@ struct Prob {
     uint32_t nrows_elem;   // element rows (block formats: rows of the last block row may be cut)
     uint32_t ncols_elem;   // element columns
     int64_t xoff, yoff;    // element offsets into the per-call x / y bases
+    int32_t ldx, ldy;      // leading dimensions when x / y come from pointer tables
 };
 
 template <typename T> struct Ptrs {
@@ -56,6 +57,8 @@ template <typename T> struct Ptrs {
     __device__ __forceinline__ T *y(uint32_t pid, const Prob &P) const {
         return ytab ? (T *)ytab[pid] + q0 : yb + P.yoff * ldy;
     }
+    __device__ __forceinline__ int lx(const Prob &P) const { return xtab ? P.ldx : ldx; }
+    __device__ __forceinline__ int ly(const Prob &P) const { return ytab ? P.ldy : ldy; }
 };
 
 // ---------------------------------------------------------------------------
@@ -138,7 +141,7 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
                                                         const uint32_t *__restrict__ crow,
                                                         const uint32_t *__restrict__ cnnz, Ptrs<T> io,
                                                         uint32_t *__restrict__ carry_row,
-                                                        acc_t<T> *__restrict__ carry_val) {
+                                                        acc_t<T> *__restrict__ carry_val, uint32_t nunits) {
     using A = acc_t<T>;
     constexpr int TILE = TPB * IPT;
     // One right-hand side: stage products (the x gather happens in the
@@ -154,12 +157,15 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     __shared__ A s_wc[K][TPB / WAVE];
 
     const int tid = threadIdx.x;
-    const uint32_t t = blockIdx.x, pid = unit_pid[t], ci = t + pid;
+    // Grid-stride over tiles: one tile per workgroup for a plain launch, a
+    // persistent sweep when the grid is capped (batched launches).
+    for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
+    const uint32_t pid = unit_pid[t], ci = t + pid;
     const Prob P = probs[pid];
     const T *__restrict__ x = io.x(pid, P);
     T *__restrict__ y = io.y(pid, P);
     const T *__restrict__ vals = (const T *)P.val;
-    const int ldx = io.ldx, ldy = io.ldy;
+    const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t i0 = crow[ci], j0 = cnnz[ci];
     const uint32_t na = crow[ci + 1] - i0, nb = cnnz[ci + 1] - j0;
 
@@ -252,6 +258,8 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
 #pragma unroll
         for (int q = 0; q < K; q++) carry_val[(size_t)t * K + q] = acc[q];
     }
+    __syncthreads();  // LDS is reused by the next tile
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -261,16 +269,18 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
 template <typename T, int L, int K>
 __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict__ probs,
                                                          const uint32_t *__restrict__ unit_pid,
-                                                         const uint32_t *__restrict__ urow, Ptrs<T> io) {
+                                                         const uint32_t *__restrict__ urow, Ptrs<T> io,
+                                                         uint32_t nunits) {
     using A = acc_t<T>;
     constexpr int G = TPB / L;
-    const uint32_t u = blockIdx.x, pid = unit_pid[u];
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint32_t pid = unit_pid[u];
     const Prob P = probs[pid];
     const T *__restrict__ x = io.x(pid, P);
     T *__restrict__ y = io.y(pid, P);
     const T *__restrict__ vals = (const T *)P.val;
     const uint32_t *__restrict__ colind = P.cind;
-    const int ldx = io.ldx, ldy = io.ldy;
+    const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
     const int lane = threadIdx.x % L, grp = threadIdx.x / L;
     for (uint32_t r = r0 + grp; r < r1; r += G) {
@@ -301,6 +311,7 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
             for (int q = 0; q < K; q++) y[(size_t)r * ldy + q] = to_val<T>(acc[q]);
         }
     }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -314,7 +325,7 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
                                                       const uint32_t *__restrict__ unit_pid,
                                                       const uint32_t *__restrict__ cent, Ptrs<T> io,
                                                       uint32_t *__restrict__ carry_row,
-                                                      acc_t<T> *__restrict__ carry_val) {
+                                                      acc_t<T> *__restrict__ carry_val, uint32_t nunits) {
     using A = acc_t<T>;
     constexpr int TILE = TPB * IPT;
     constexpr bool PROD = (K == 1);
@@ -327,12 +338,13 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     __shared__ A s_wc[K][TPB / WAVE];
 
     const int tid = threadIdx.x;
-    const uint32_t t = blockIdx.x, pid = unit_pid[t], ci = t + pid;
+    for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
+    const uint32_t pid = unit_pid[t], ci = t + pid;
     const Prob P = probs[pid];
     const T *__restrict__ x = io.x(pid, P);
     T *__restrict__ y = io.y(pid, P);
     const T *__restrict__ vals = (const T *)P.val;
-    const int ldx = io.ldx, ldy = io.ldy;
+    const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t e0 = cent[ci], n = cent[ci + 1] - e0;
     const uint32_t next_row = (e0 + n < P.nent) ? P.rind[e0 + n] : NONE;
 
@@ -432,6 +444,8 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
 #pragma unroll
         for (int q = 0; q < K; q++) carry_val[(size_t)t * K + q] = acc[q];
     }
+    __syncthreads();  // LDS is reused by the next tile
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -441,15 +455,17 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
 template <typename T, int R, int C, int L, int K>
 __global__ __launch_bounds__(TPB) void bcsr_vector_kernel(const Prob *__restrict__ probs,
                                                           const uint32_t *__restrict__ unit_pid,
-                                                          const uint32_t *__restrict__ urow, Ptrs<T> io) {
+                                                          const uint32_t *__restrict__ urow, Ptrs<T> io,
+                                                         uint32_t nunits) {
     using A = acc_t<T>;
     constexpr int G = TPB / L;
-    const uint32_t u = blockIdx.x, pid = unit_pid[u];
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint32_t pid = unit_pid[u];
     const Prob P = probs[pid];
     const T *__restrict__ x = io.x(pid, P);
     T *__restrict__ y = io.y(pid, P);
     const T *__restrict__ bv = (const T *)P.val;
-    const int ldx = io.ldx, ldy = io.ldy;
+    const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
     const int lane = threadIdx.x % L, grp = threadIdx.x / L;
     const uint32_t ncols = P.ncols_elem;
@@ -493,6 +509,7 @@ __global__ __launch_bounds__(TPB) void bcsr_vector_kernel(const Prob *__restrict
             }
         }
     }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -518,12 +535,13 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
     __shared__ A s_wc[R][TPB / WAVE];
 
     const int tid = threadIdx.x;
-    const uint32_t t = blockIdx.x, pid = unit_pid[t], ci = t + pid;
+    for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
+    const uint32_t pid = unit_pid[t], ci = t + pid;
     const Prob P = probs[pid];
     const T *__restrict__ x = io.x(pid, P);
     T *__restrict__ y = io.y(pid, P);
     const T *__restrict__ bv = (const T *)P.val;
-    const int ldy = io.ldy, ldx = io.ldx;
+    const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t b0 = cent[ci], n = cent[ci + 1] - b0;
     const uint32_t next_brow = (b0 + n < P.nent) ? P.rind[b0 + n] : NONE;
 
@@ -618,6 +636,8 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
             carry_val[(size_t)i * nunits + t] = acc[i];
         }
     }
+    __syncthreads();  // LDS is reused by the next tile
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -650,7 +670,7 @@ __global__ void fixup_kernel(const Prob *__restrict__ probs, const uint32_t *__r
         const Prob P = probs[pid];
         const uint32_t row = r * S + s;
         if (row >= P.nrows_elem) return;
-        T *yp = io.y(pid, P) + (size_t)row * io.ldy;
+        T *yp = io.y(pid, P) + (size_t)row * io.ly(P);
 #pragma unroll
         for (int q = 0; q < K; q++) yp[q] = to_val<T>((A)yp[q] + sum[q]);
         return;
@@ -659,7 +679,7 @@ __global__ void fixup_kernel(const Prob *__restrict__ probs, const uint32_t *__r
     if (z >= nzero) return;
     const uint32_t pid = zpid[z];
     const Prob P = probs[pid];
-    T *yp = io.y(pid, P) + (size_t)zrow[z] * io.ldy;
+    T *yp = io.y(pid, P) + (size_t)zrow[z] * io.ly(P);
 #pragma unroll
     for (int q = 0; q < K; q++) yp[q] = T(0);
 }

@@ -11,20 +11,20 @@ namespace spd {
 template <typename T, int K>
 static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
     using A = acc_t<T>;
-    const dim3 grid(w.nunits), block(TPB);
+    const dim3 grid(w.grid_cap && w.grid_cap < w.nunits ? w.grid_cap : w.nunits), block(TPB);
     A *cv = (A *)w.d_carry_val;
     switch (w.kind) {
     case WK_CSR_MERGE:
         if (K > 1 && w.ipt > 8) return SP_ERR_UNSUPPORTED;
         switch (w.ipt) {
-        case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv); break;
-        case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv); break;
-        case 16: hipLaunchKernelGGL((csr_merge_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv); break;
+        case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+        case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+        case 16: hipLaunchKernelGGL((csr_merge_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
         default: return SP_ERR_ARG;
         }
         break;
     case WK_CSR_VEC:
-#define SPD_VEC(Lv) case Lv: hipLaunchKernelGGL((csr_vector_kernel<T, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io); break;
+#define SPD_VEC(Lv) case Lv: hipLaunchKernelGGL((csr_vector_kernel<T, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
         switch (w.lanes) {
             SPD_VEC(1) SPD_VEC(2) SPD_VEC(4) SPD_VEC(8) SPD_VEC(16) SPD_VEC(32) SPD_VEC(64)
         default: return SP_ERR_ARG;
@@ -34,15 +34,15 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
     case WK_COO_SEG:
         if (K > 1 && w.ipt > 8) return SP_ERR_UNSUPPORTED;
         switch (w.ipt) {
-        case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv); break;
-        case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv); break;
-        case 16: hipLaunchKernelGGL((coo_seg_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv); break;
+        case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+        case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+        case 16: hipLaunchKernelGGL((coo_seg_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
         default: return SP_ERR_ARG;
         }
         break;
     case WK_BCSR_VEC: {
 #define SPD_BL(Rv, Cv, Lv) \
-    if (w.R == Rv && w.C == Cv && w.lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io); break; }
+    if (w.R == Rv && w.C == Cv && w.lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break; }
 #define SPD_BRC(Rv, Cv) SPD_BL(Rv, Cv, 1) SPD_BL(Rv, Cv, 4) SPD_BL(Rv, Cv, 16) SPD_BL(Rv, Cv, 64)
         do {
             SPD_BRC(1, 1) SPD_BRC(2, 2) SPD_BRC(4, 4) SPD_BRC(8, 8) SPD_BRC(2, 4) SPD_BRC(4, 2)
@@ -92,6 +92,27 @@ static int launch_chunk(const Work &w, sp_ctx *ctx, Ptrs<T> io) {
 }
 
 }  // namespace spd
+
+// One launch group whose problems take x / y from device pointer tables and
+// their leading dimensions from the descriptors; kc right-hand sides.
+template <typename T>
+int run_tables_t(const Work &w, sp_ctx *ctx, const uintptr_t *xtab, const uintptr_t *ytab, int kc) {
+    using namespace spd;
+    Ptrs<T> io;
+    io.xb = nullptr;
+    io.yb = nullptr;
+    io.xtab = xtab;
+    io.ytab = ytab;
+    io.ldx = io.ldy = 0;
+    io.q0 = 0;
+    switch (kc) {
+    case 8: return launch_chunk<T, 8>(w, ctx, io);
+    case 4: return launch_chunk<T, 4>(w, ctx, io);
+    case 2: return launch_chunk<T, 2>(w, ctx, io);
+    case 1: return launch_chunk<T, 1>(w, ctx, io);
+    default: return SP_ERR_ARG;
+    }
+}
 
 template <typename T>
 int run_work_t(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintptr_t *xtab,

@@ -63,6 +63,12 @@ struct Builder {
             if ((rc = plan_alloc(p, n * w.carry_k * acc_size(p->dtype), &w.d_carry_val))) return rc;
         }
         p->launches = 1 + ((carries || w.nzero) ? 1 : 0);
+        p->h_probs = probs;
+        p->h_unit_pid = unit_pid;
+        p->h_c0 = c0;
+        p->h_c1 = c1;
+        p->h_zpid = zpid;
+        p->h_zrow = zrow;
         return SP_OK;
     }
 };

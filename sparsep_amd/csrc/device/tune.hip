@@ -1,0 +1,165 @@
+// Exhaustive autotuner: enumerates the variant space (the 25 kernel ids and
+// their GPU knobs), builds and times every point with HIP events, and keeps
+// a JSON-lines timing database.  The learned selector is trained on these
+// timings.  No counterpart in the reference.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace {
+
+struct Pt {
+    int id;
+    sp_opts o;
+};
+
+void add(std::vector<Pt> &v, int id, const sp_opts &base, void (*f)(sp_opts &, int), int arg) {
+    Pt p{id, base};
+    f(p.o, arg);
+    v.push_back(p);
+}
+
+}  // namespace
+
+// Candidate points of one kernel id for matrix A (row statistics decide the
+// lane counts worth trying).
+static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
+    sp_opts base;
+    sp_opts_default(&base);
+    const double avg = A->nrows ? (double)A->nnz / A->nrows : 1.0;
+    uint32_t Lc = 1;
+    while (Lc < 64 && Lc * 2 <= avg * 1.5) Lc *= 2;
+    if (id == SP_K_CSR_ROW || id == SP_K_CSR_NNZ) {
+        for (uint32_t L : {Lc / 2, Lc, Lc * 2}) {
+            if (L < 1 || L > 64) continue;
+            Pt p{id, base};
+            p.o.algo = SP_ALGO_VECTOR;
+            p.o.lanes_per_row = L;
+            v.push_back(p);
+        }
+        for (uint32_t I : {4u, 8u, 16u}) {
+            Pt p{id, base};
+            p.o.algo = SP_ALGO_SEGMENTED;
+            p.o.items_per_lane = I;
+            v.push_back(p);
+        }
+    } else if (id >= SP_K_COO_ROW && id <= SP_K_COO_NNZ) {
+        for (uint32_t I : {4u, 8u, 16u}) {
+            Pt p{id, base};
+            p.o.items_per_lane = I;
+            v.push_back(p);
+        }
+    } else if (id >= SP_K_BCSR_BLOCK && id <= SP_K_BCOO_NNZ) {
+        for (uint32_t r : {2u, 4u}) {
+            Pt p{id, base};
+            p.o.row_block_size = p.o.col_block_size = r;
+            v.push_back(p);
+        }
+    } else {
+        for (uint32_t vert : {2u, 4u, 8u}) {
+            Pt p{id, base};
+            p.o.vert = vert;
+            v.push_back(p);
+        }
+    }
+    (void)add;
+}
+
+extern "C" int sp_tune_space(const sp_coo *A, int kernel_id, sp_tune_point *pts, int cap, int *count) {
+    if (!A || !count) return SP_ERR_ARG;
+    std::vector<Pt> v;
+    for (int id = 0; id < SP_K_COUNT; id++)
+        if (kernel_id < 0 || kernel_id == id) space_for(A, id, v);
+    *count = (int)v.size();
+    if (!pts) return SP_OK;
+    if (cap < (int)v.size()) return SP_ERR_ARG;
+    for (size_t i = 0; i < v.size(); i++) {
+        memset(&pts[i], 0, sizeof(pts[i]));
+        pts[i].kernel_id = v[i].id;
+        pts[i].opts = v[i].o;
+        pts[i].ms = -1.f;
+        pts[i].status = 1;  // not run
+    }
+    return SP_OK;
+}
+
+extern "C" int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters,
+                           uint64_t flush_bytes) {
+    if (!ctx || !A || !pts || n < 0 || iters <= 0) return SP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    const size_t sv = sp_dtype_size(A->dtype);
+    void *x = nullptr, *y = nullptr;
+    if (hipMalloc(&x, (size_t)std::max<uint32_t>(A->ncols, 1) * sv) != hipSuccess) return SP_ERR_NOMEM;
+    if (hipMalloc(&y, (size_t)std::max<uint32_t>(A->nrows, 1) * sv) != hipSuccess) { (void)hipFree(x); return SP_ERR_NOMEM; }
+    {
+        std::vector<char> hx((size_t)std::max<uint32_t>(A->ncols, 1) * sv);
+        sp_dist d;
+        d.kind = SP_DIST_UNIFORM;
+        d.lo = -1;
+        d.hi = 1;
+        sp_gen_vector(A->dtype, A->ncols, d, 12345, hx.data());
+        (void)hipMemcpy(x, hx.data(), hx.size(), hipMemcpyHostToDevice);
+    }
+    std::vector<float> ms(iters);
+    for (int i = 0; i < n; i++) {
+        sp_tune_point &pt = pts[i];
+        sp_plan *p = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = sp_plan_create(ctx, pt.kernel_id, A, &pt.opts, &p);
+        const auto t1 = std::chrono::steady_clock::now();
+        pt.build_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
+        if (rc) {
+            pt.status = rc;
+            pt.ms = -1.f;
+            continue;
+        }
+        rc = sp_time_spmv(p, x, y, 1, flush_bytes, ms.data());  // warm-up
+        if (!rc) rc = sp_time_spmv(p, x, y, iters, flush_bytes, ms.data());
+        if (!rc) {
+            std::vector<float> s(ms);
+            std::sort(s.begin(), s.end());
+            pt.ms = s[s.size() / 2];
+            sp_plan_info info;
+            sp_plan_get_info(p, &info);
+            pt.algo_bytes = info.algo_bytes;
+        }
+        pt.status = rc;
+        sp_plan_destroy(p);
+    }
+    (void)hipFree(x);
+    (void)hipFree(y);
+    return SP_OK;
+}
+
+extern "C" int sp_tune_best(const sp_tune_point *pts, int n, int kernel_id) {
+    int best = -1;
+    for (int i = 0; i < n; i++) {
+        if (pts[i].status != 0 || pts[i].ms < 0) continue;
+        if (kernel_id >= 0 && pts[i].kernel_id != kernel_id) continue;
+        if (best < 0 || pts[i].ms < pts[best].ms) best = i;
+    }
+    return best;
+}
+
+extern "C" int sp_tune_db_append(const char *path, const char *tag, const sp_tune_point *pts, int n) {
+    if (!path || !pts) return SP_ERR_ARG;
+    FILE *f = fopen(path, "a");
+    if (!f) return SP_ERR_IO;
+    for (int i = 0; i < n; i++) {
+        const sp_tune_point &p = pts[i];
+        fprintf(f,
+                "{\"matrix\": \"%s\", \"kernel\": %d, \"name\": \"%s\", \"algo\": %d, \"lanes\": %u, \"ipt\": %u, "
+                "\"vert\": %u, \"r\": %u, \"c\": %u, \"nparts\": %u, \"nsub\": %u, \"ms\": %.6f, "
+                "\"build_ms\": %.3f, \"status\": %d}\n",
+                tag ? tag : "", p.kernel_id, sp_kernel_name(p.kernel_id), p.opts.algo, p.opts.lanes_per_row,
+                p.opts.items_per_lane, p.opts.vert, p.opts.row_block_size, p.opts.col_block_size, p.opts.nparts,
+                p.opts.nsub, p.ms, p.build_ms, p.status);
+    }
+    fclose(f);
+    return SP_OK;
+}

@@ -28,6 +28,7 @@ struct Work {
     uint32_t *d_zpid = nullptr, *d_zrow = nullptr;
     bool zero_all = false;  // zero every output row first (2D partial buffers)
     uint64_t zero_rows = 0; // rows to zero when zero_all (times ldy elements)
+    uint32_t grid_cap = 0;  // persistent launch: at most this many workgroups (0 = one per unit)
 };
 
 // Launches `w` for k right-hand sides (processed in chunks of 8/4/2/1).

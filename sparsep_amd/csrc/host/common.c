@@ -101,6 +101,14 @@ int sp_gen_vector(int dtype, uint64_t n, sp_dist d, uint64_t seed, void *out) {
 
 void sp_radix_sort_u64(uint64_t *keys, uint64_t n) {
     if (n < 2) return;
+    if (n <= 256) { /* a 64K-bucket pass costs far more than insertion sort here */
+        for (uint64_t i = 1; i < n; i++) {
+            uint64_t v = keys[i], j = i;
+            while (j > 0 && keys[j - 1] > v) { keys[j] = keys[j - 1]; j--; }
+            keys[j] = v;
+        }
+        return;
+    }
     uint64_t *tmp = (uint64_t *)malloc(n * sizeof(uint64_t));
     if (!tmp) { /* fall back to an in-place shell sort */
         for (uint64_t gap = n / 2; gap > 0; gap /= 2)
