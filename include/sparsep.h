@@ -567,6 +567,25 @@ int sp_mlp_adam(sp_mlp *m, float lr, float beta1, float beta2, float eps);
 int sp_mlp_train(sp_mlp *m, const float *X, const float *T, const float *M, int n, int epochs,
                  int batch, float lr, uint64_t seed, float *final_loss);
 
+/* The selector: an MLP SP_NFEAT -> h1 -> h2 -> SP_K_COUNT over standardised
+ * features ((f - mean) / std) whose outputs score each kernel id (trained as
+ * log2 of the id's default-configuration time over the exhaustive best, so
+ * lower is better).  sp_select picks the lowest score with default options;
+ * when the gap to the second-lowest score is below `margin` and
+ * fallback_k > 0, it instead tunes every point of the fallback_k
+ * lowest-scored ids (sp_tune_run with iters / flush_bytes) and returns the
+ * fastest (*fell_back = 1).  scores (SP_K_COUNT floats) may be NULL.
+ * Model files are written by sp_selector_save (magic "SPSEL001"). */
+typedef struct sp_selector sp_selector;
+int sp_selector_create(sp_ctx *ctx, const int *sizes, const float *params, size_t nparams,
+                       const float *feat_mean, const float *feat_std, float margin,
+                       sp_selector **s);
+int sp_selector_load(sp_ctx *ctx, const char *path, sp_selector **s);
+int sp_selector_save(const sp_selector *s, const char *path);
+int sp_selector_destroy(sp_selector *s);
+int sp_select(sp_selector *s, const sp_coo *A, int fallback_k, int iters, uint64_t flush_bytes,
+              sp_tune_point *choice, float *scores, int *fell_back);
+
 #ifdef __cplusplus
 }
 #endif

@@ -183,6 +183,12 @@ SIGNATURES = {
     "sp_mlp_loss_grad": (I, [P, fp, fp, fp, I, fp, fp]),
     "sp_mlp_adam": (I, [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     "sp_mlp_train": (I, [P, fp, fp, fp, I, I, I, ctypes.c_float, U64, fp]),
+    "sp_selector_create": (I, [P, ctypes.POINTER(ctypes.c_int), fp, S, fp, fp, ctypes.c_float, PP]),
+    "sp_selector_load": (I, [P, ctypes.c_char_p, PP]),
+    "sp_selector_save": (I, [P, ctypes.c_char_p]),
+    "sp_selector_destroy": (I, [P]),
+    "sp_select": (I, [P, ctypes.POINTER(sp_coo), I, I, U64, ctypes.POINTER(sp_tune_point), fp,
+                      ctypes.POINTER(ctypes.c_int)]),
 }
 
 

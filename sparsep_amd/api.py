@@ -638,3 +638,38 @@ class Mlp:
         check(lib().sp_mlp_train(self.ptr, _fptr(X), _fptr(T), _fptr(M), X.shape[0], epochs, batch,
                                  lr, seed, ctypes.byref(loss)), "sp_mlp_train")
         return loss.value
+
+
+class Selector:
+    """Learned variant selector (sp_selector): load a model file or build one
+    from trained MLP parameters and the feature standardisation."""
+
+    def __init__(self, ctx, path=None, params=None, mean=None, std=None, margin=0.0,
+                 sizes=(NFEAT, 128, 128, 25)):
+        self.ctx = ctx
+        self.ptr = ctypes.c_void_p()
+        if path is not None:
+            check(lib().sp_selector_load(ctx.ptr, str(path).encode(), ctypes.byref(self.ptr)),
+                  f"sp_selector_load({path})")
+        else:
+            sz = (ctypes.c_int * 4)(*sizes)
+            params, mean, std = _f32(params), _f32(mean), _f32(std)
+            check(lib().sp_selector_create(ctx.ptr, sz, _fptr(params), params.size, _fptr(mean), _fptr(std),
+                                           margin, ctypes.byref(self.ptr)), "sp_selector_create")
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib().sp_selector_destroy(self.ptr)
+            self.ptr = None
+
+    def save(self, path):
+        check(lib().sp_selector_save(self.ptr, str(path).encode()), "sp_selector_save")
+
+    def select(self, A, fallback_k=0, iters=5, flush_bytes=0):
+        """Returns (tune point, scores[25], fell_back)."""
+        pt = _lib.sp_tune_point()
+        scores = np.zeros(len(KERNELS), np.float32)
+        fb = ctypes.c_int()
+        check(lib().sp_select(self.ptr, A.ptr, fallback_k, iters, flush_bytes, ctypes.byref(pt), _fptr(scores),
+                              ctypes.byref(fb)), "sp_select")
+        return pt, scores, bool(fb.value)

@@ -6,7 +6,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <vector>
 
 #include "plan.hpp"
@@ -18,19 +20,16 @@ struct Pt {
     sp_opts o;
 };
 
-void add(std::vector<Pt> &v, int id, const sp_opts &base, void (*f)(sp_opts &, int), int arg) {
-    Pt p{id, base};
-    f(p.o, arg);
-    v.push_back(p);
-}
-
 }  // namespace
 
-// Candidate points of one kernel id for matrix A (row statistics decide the
-// lane counts worth trying).
+// Candidate points of one kernel id for matrix A.  The first point of every
+// id is its default configuration (sp_opts_default: automatic variant and
+// knob choice), the one the learned selector runs; the others vary one knob
+// each (row statistics decide the lane counts worth trying).
 static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
     sp_opts base;
     sp_opts_default(&base);
+    v.push_back(Pt{id, base});
     const double avg = A->nrows ? (double)A->nnz / A->nrows : 1.0;
     uint32_t Lc = 1;
     while (Lc < 64 && Lc * 2 <= avg * 1.5) Lc *= 2;
@@ -49,25 +48,22 @@ static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
             v.push_back(p);
         }
     } else if (id >= SP_K_COO_ROW && id <= SP_K_COO_NNZ) {
-        for (uint32_t I : {4u, 8u, 16u}) {
+        for (uint32_t I : {4u, 16u}) {
             Pt p{id, base};
             p.o.items_per_lane = I;
             v.push_back(p);
         }
     } else if (id >= SP_K_BCSR_BLOCK && id <= SP_K_BCOO_NNZ) {
-        for (uint32_t r : {2u, 4u}) {
-            Pt p{id, base};
-            p.o.row_block_size = p.o.col_block_size = r;
-            v.push_back(p);
-        }
+        Pt p{id, base};
+        p.o.row_block_size = p.o.col_block_size = 2;
+        v.push_back(p);
     } else {
-        for (uint32_t vert : {2u, 4u, 8u}) {
+        for (uint32_t vert : {4u, 8u}) {
             Pt p{id, base};
             p.o.vert = vert;
             v.push_back(p);
         }
     }
-    (void)add;
 }
 
 extern "C" int sp_tune_space(const sp_coo *A, int kernel_id, sp_tune_point *pts, int cap, int *count) {
@@ -88,6 +84,10 @@ extern "C" int sp_tune_space(const sp_coo *A, int kernel_id, sp_tune_point *pts,
     return SP_OK;
 }
 
+// Builds run on host threads (plan creation is host work plus synchronous
+// uploads, safe to overlap), up to `kBatch` plans at a time; the timed SpMVs
+// then run one plan after another on the context stream, so timings never
+// overlap.
 extern "C" int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters,
                            uint64_t flush_bytes) {
     if (!ctx || !A || !pts || n < 0 || iters <= 0) return SP_ERR_ARG;
@@ -105,31 +105,52 @@ extern "C" int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int
         sp_gen_vector(A->dtype, A->ncols, d, 12345, hx.data());
         (void)hipMemcpy(x, hx.data(), hx.size(), hipMemcpyHostToDevice);
     }
+    constexpr int kBatch = 16;
+    const int nthreads = (int)std::max(1u, std::min<unsigned>(kBatch, std::thread::hardware_concurrency()));
     std::vector<float> ms(iters);
-    for (int i = 0; i < n; i++) {
-        sp_tune_point &pt = pts[i];
-        sp_plan *p = nullptr;
-        const auto t0 = std::chrono::steady_clock::now();
-        int rc = sp_plan_create(ctx, pt.kernel_id, A, &pt.opts, &p);
-        const auto t1 = std::chrono::steady_clock::now();
-        pt.build_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
-        if (rc) {
+    for (int b0 = 0; b0 < n; b0 += kBatch) {
+        const int b1 = std::min(n, b0 + kBatch);
+        std::vector<sp_plan *> plans(b1 - b0, nullptr);
+        std::vector<int> rcs(b1 - b0, 0);
+        std::atomic<int> next(b0);
+        auto worker = [&]() {
+            (void)hipSetDevice(ctx->device);
+            for (int i; (i = next.fetch_add(1)) < b1;) {
+                sp_tune_point &pt = pts[i];
+                const auto t0 = std::chrono::steady_clock::now();
+                rcs[i - b0] = sp_plan_create(ctx, pt.kernel_id, A, &pt.opts, &plans[i - b0]);
+                const auto t1 = std::chrono::steady_clock::now();
+                pt.build_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 0; t < std::min(nthreads, b1 - b0); t++) pool.emplace_back(worker);
+        for (auto &t : pool) t.join();
+        for (int i = b0; i < b1; i++) {
+            sp_tune_point &pt = pts[i];
+            sp_plan *p = plans[i - b0];
+            int rc = rcs[i - b0];
+            if (rc) {
+                pt.status = rc;
+                pt.ms = -1.f;
+                continue;
+            }
+            rc = sp_time_spmv(p, x, y, 1, flush_bytes, ms.data());  // warm-up
+            if (!rc) rc = sp_time_spmv(p, x, y, iters, flush_bytes, ms.data());
+            if (!rc) {
+                std::vector<float> s(ms);
+                std::sort(s.begin(), s.end());
+                pt.ms = s[s.size() / 2];
+                sp_plan_info info;
+                sp_plan_get_info(p, &info);
+                pt.algo_bytes = info.algo_bytes;
+            } else {
+                pt.ms = -1.f;
+            }
             pt.status = rc;
-            pt.ms = -1.f;
-            continue;
         }
-        rc = sp_time_spmv(p, x, y, 1, flush_bytes, ms.data());  // warm-up
-        if (!rc) rc = sp_time_spmv(p, x, y, iters, flush_bytes, ms.data());
-        if (!rc) {
-            std::vector<float> s(ms);
-            std::sort(s.begin(), s.end());
-            pt.ms = s[s.size() / 2];
-            sp_plan_info info;
-            sp_plan_get_info(p, &info);
-            pt.algo_bytes = info.algo_bytes;
-        }
-        pt.status = rc;
-        sp_plan_destroy(p);
+        for (sp_plan *p : plans)
+            if (p) sp_plan_destroy(p);
     }
     (void)hipFree(x);
     (void)hipFree(y);
