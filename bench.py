@@ -1,13 +1,13 @@
 #!/usr/bin/env python3
 """SpMV bench on one MI355X.
 
-Workload: the seeded R-MAT matrix of 2^20 x 2^20 with 2^24 unique entries
-(quadrants a=0.57, b=0.19, c=0.19; values and x uniform in [-1,1]; seed 2),
-FP32, CSR with the entry-balanced (merge-path) kernel.  One step = one
-SpMV y = A x with A, x, y resident in HBM.  Before every step a 1 GiB
-buffer is overwritten so that L2 and the 256 MB MALL hold nothing of A or x
-(the 147 MB working set would otherwise be served from MALL); the flush is
-outside the timed events.
+Default workload (--workload rmat): the seeded R-MAT matrix of 2^20 x 2^20
+with 2^24 unique entries (quadrants a=0.57, b=0.19, c=0.19; values and x
+uniform in [-1,1]; seed 2), FP32, CSR with the entry-balanced (merge-path)
+kernel.  One step = one SpMV y = A x with A, x, y resident in HBM.  Before
+every step a 1 GiB buffer is overwritten so that L2 and the 256 MB MALL hold
+nothing of A or x (the 147 MB working set would otherwise be served from
+MALL); the flush is outside the timed events.
 
 value = algorithmic bytes of one SpMV (nnz*(4+4) + (n+1)*4 + (m+n)*4,
 x and y counted once) / mean SpMV time measured with HIP events on the
@@ -15,9 +15,19 @@ library's stream.  roofline.achieved uses the duration of the dominant
 (matrix-streaming) kernel alone.  cpu_baseline times the oracle's C
 restatement of SparseP's CSR spmv_host (single thread, reference order) on
 the same matrix and vector, on this host.
+
+Other workloads print one JSON line each in the same format:
+  stencil5   256x256 5-point Laplacian, INT32, CSR-row, bit-exact check
+  formats    the R-MAT matrix in every 1D format and every dtype
+  stencil27  64^3 x 4-DOF 27-point stencil, 2D kernels, FP32 and FP64
+  batched    256 uneven SpMV/SpMM problems, one persistent hipGraph launch
+             against one launch per problem, FP32 and INT16
+  selector   learned variant choice against a live exhaustive search on
+             held-out corpus matrices
 """
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -28,45 +38,49 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SpMV GB/s & fraction of HBM peak; learned-pick vs exhaustive-best ratio"
 HBM_PEAK_GBS = 8000.0
+DT_NAME = {0: "i8", 1: "i16", 2: "i32", 3: "i64", 4: "f32", 5: "f64"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--flush-mb", type=int, default=1024)
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline timing")
-    ap.add_argument("--scale", type=int, default=20)
-    ap.add_argument("--nnz-log2", type=int, default=24)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
+def line(args, value, ms, dtype, config, unit="GB/s", roofline=None, cpu=None, extra=None, data=None,
+         higher=True):
+    return {
+        "metric": METRIC, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 6), "higher_is_better": higher, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": data or "synthetic (seeded generators, project PRNG)",
+        "config": config, "roofline": roofline, "cpu_baseline": cpu, "extra": extra or {},
+    }
 
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only: this build targets a single MI355X and has no "
-              f"multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
-        sys.exit(2)
 
+def measure(sp, plan, xd, yd, args, flush):
+    """Untimed warm-up, then `steps` timed SpMVs with the dominant kernel split."""
+    if args.warmup > 0:
+        plan.time_spmv(xd.addr, yd.addr, args.warmup, flush)
+    ms, md = plan.time_spmv_split(xd.addr, yd.addr, args.steps, flush)
+    return statistics.mean(ms), statistics.mean(md)
+
+
+def parity_ok(orc, dt, A, x, y):
     import numpy as np
+    C = A.to_csr()
+    if dt <= 3:
+        return bool(np.array_equal(y, orc.spmv_csr(dt, C.rowptr, C.colind, C.values, x)))
+    yref, absb = orc.ref_csr(dt, C.rowptr, C.colind, C.values, x)
+    tol = orc.fp_tolerance(dt, np.diff(C.rowptr.astype(np.int64)), absb)
+    return bool(np.all(np.abs(y.astype(np.float64) - yref) <= tol))
 
-    import sparsep_amd.api as sp
 
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-
-    ctx = sp.Context(0)
-    info = ctx.info()
-    log(f"[bench] device {info['arch']} CUs={info['num_cus']} HBM={info['hbm_bytes'] / 2**30:.0f} GiB")
-
+# --------------------------------------------------------------------------
+def wl_rmat(sp, ctx, args, orc):
+    import numpy as np
     t0 = time.time()
     dt = sp.F32
     A = sp.gen_rmat(args.scale, 1 << args.nnz_log2, 0.57, 0.19, 0.19, dtype=dt, seed=2)
     x = sp.gen_vector(dt, A.ncols, seed=22)
     log(f"[bench] R-MAT 2^{args.scale}: nnz={A.nnz} generated in {time.time() - t0:.1f}s")
-
     plan = sp.Plan(ctx, "CSR-nnz", A, sp.opts(algo=sp.ALGO_SEGMENTED, items_per_lane=8))
     pinfo = plan.info()
     algo_bytes = pinfo["algo_bytes"]
@@ -74,42 +88,22 @@ def main():
     yd = sp.DeviceArray(ctx, A.nrows, np.float32)
     flush = args.flush_mb << 20
     ctx.sync()
-
-    # warmup (untimed)
-    if args.warmup > 0:
-        plan.time_spmv(xd.addr, yd.addr, args.warmup, flush)
-    ctx.sync()
     t1 = time.perf_counter()
-    ms, ms_dom = plan.time_spmv_split(xd.addr, yd.addr, args.steps, flush)
-    ctx.sync()
+    mean_ms, mean_dom = measure(sp, plan, xd, yd, args, flush)
     wall = time.perf_counter() - t1
-    mean_ms = statistics.mean(ms)
-    mean_dom = statistics.mean(ms_dom)
     value = algo_bytes / (mean_ms * 1e-3) / 1e9
     achieved = algo_bytes / (mean_dom * 1e-3) / 1e9
-    log(f"[bench] SpMV mean {mean_ms * 1e3:.2f} us (median {statistics.median(ms) * 1e3:.2f}), "
-        f"dominant kernel {mean_dom * 1e3:.2f} us, {value:.0f} GB/s, wall {wall:.2f}s")
-
-    # warm-cache (MALL-resident) figure, reported separately
+    log(f"[bench] SpMV mean {mean_ms * 1e3:.2f} us, dominant kernel {mean_dom * 1e3:.2f} us, "
+        f"{value:.0f} GB/s, wall {wall:.2f}s")
     ms_warm = plan.time_spmv(xd.addr, yd.addr, 20, 0)
     warm_gbs = algo_bytes / (statistics.median(ms_warm) * 1e-3) / 1e9
-
-    # stream-copy probe: 1 GiB read + 1 GiB write
     cms = ctx.time_copy(1 << 30, 10)
     copy_gbs = 2 * (1 << 30) / (statistics.median(cms[2:]) * 1e-3) / 1e9
-
-    # parity of the bench output against the oracle (float tolerance)
-    import oracle_lib as orc
-
-    y = yd.numpy()
-    C = A.to_csr()
-    yref, absb = orc.ref_csr(dt, C.rowptr, C.colind, C.values, x)
-    tol = orc.fp_tolerance(dt, np.diff(C.rowptr.astype(np.int64)), absb)
-    parity = bool(np.all(np.abs(y.astype(np.float64) - yref) <= tol))
+    parity = parity_ok(orc, dt, A, x, yd.numpy())
     log(f"[bench] parity vs oracle: {parity}")
-
     cpu = None
     if not args.no_cpu:
+        C = A.to_csr()
         cms_cpu, iters, _ = orc.time_csr(dt, C.rowptr, C.colind, C.values, x, iters=30, warmup=2,
                                         omp_threads=0, budget_s=args.cpu_budget)
         cpu_gbs = algo_bytes / (cms_cpu * 1e-3) / 1e9
@@ -118,35 +112,253 @@ def main():
                          f"spmv_host (spmv/1D/CSR-row/host/app.c:66-77), 1 thread, {cms_cpu:.1f} ms/run",
                "host_cpus": orc.num_cpus()}
         log(f"[bench] cpu baseline {cms_cpu:.1f} ms/run ({cpu_gbs:.2f} GB/s, {iters} runs)")
+    cfg = {"workload": f"R-MAT 2^{args.scale} x 2^{args.scale}, {A.nnz} unique nnz, a=0.57 b=0.19 c=0.19, "
+                       f"FP32 CSR SpMV (CSR-nnz, merge-path kernel), MALL flushed before each step",
+           "nrows": A.nrows, "nnz": A.nnz, "algo_bytes": algo_bytes,
+           "gflops": round(2 * A.nnz / (mean_ms * 1e-3) / 1e9, 2)}
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+    extra = {"frac_of_peak_whole_spmv": round(value / HBM_PEAK_GBS, 4), "measured_copy_gbs": round(copy_gbs, 1),
+             "frac_of_measured_copy": round(value / copy_gbs, 4), "warm_cache_gbs": round(warm_gbs, 1),
+             "kernels_per_spmv": pinfo["launches"], "parity_vs_oracle": parity}
+    return [line(args, value, mean_ms, "f32", cfg, roofline=roof, cpu=cpu, extra=extra,
+                 data="synthetic (seeded R-MAT, project PRNG)")]
 
-    out = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(mean_ms, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded R-MAT, project PRNG)",
-        "config": {"workload": f"R-MAT 2^{args.scale} x 2^{args.scale}, {A.nnz} unique nnz, "
-                               f"a=0.57 b=0.19 c=0.19, FP32 CSR SpMV (CSR-nnz, merge-path kernel), "
-                               f"MALL flushed before each step",
-                   "nrows": A.nrows, "nnz": A.nnz, "algo_bytes": algo_bytes,
-                   "gflops": round(2 * A.nnz / (mean_ms * 1e-3) / 1e9, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
-        "cpu_baseline": cpu,
-        "extra": {"frac_of_peak_whole_spmv": round(value / HBM_PEAK_GBS, 4),
-                  "measured_copy_gbs": round(copy_gbs, 1),
-                  "frac_of_measured_copy": round(value / copy_gbs, 4),
-                  "warm_cache_gbs": round(warm_gbs, 1),
-                  "kernels_per_spmv": pinfo["launches"], "parity_vs_oracle": parity},
-    }
-    print(json.dumps(out), flush=True)
+
+def wl_stencil5(sp, ctx, args, orc):
+    import numpy as np
+    dt = sp.I32
+    A = sp.gen_stencil5(256, 256, dt, offdiag=sp.dist(sp.DIST_UNIFORM, -8, 8), diag=4.0, seed=1)
+    x = sp.gen_vector(dt, A.ncols, d=sp.dist(sp.DIST_UNIFORM, -4, 4), seed=1)
+    plan = sp.Plan(ctx, "CSR-row", A)
+    algo = plan.info()["algo_bytes"]
+    xd = sp.DeviceArray.from_numpy(ctx, x)
+    yd = sp.DeviceArray(ctx, A.nrows, np.int32)
+    mean_ms, mean_dom = measure(sp, plan, xd, yd, args, 0)
+    exact = parity_ok(orc, dt, A, x, yd.numpy())
+    cpu = None
+    if not args.no_cpu:
+        C = A.to_csr()
+        cms, iters, _ = orc.time_csr(dt, C.rowptr, C.colind, C.values, x, iters=200, warmup=5, omp_threads=0,
+                                    budget_s=5.0)
+        cpu = {"value": round(cms * 1e3, 2), "unit": "us", "cores": 1, "kind": "port",
+               "sample": f"full matrix, median of {iters} runs of the C restatement of CSR spmv_host"}
+    cfg = {"workload": "5-point Laplacian 256x256 (65,536 rows, 326,656 nnz), INT32 CSR-row SpMV, "
+                       "offdiag U{-8..8}, diag 4, x U{-4..4}, seed 1; cache-resident (no flush)",
+           "nrows": A.nrows, "nnz": A.nnz, "algo_bytes": algo}
+    return [line(args, mean_ms * 1e3, mean_ms, "i32", cfg, unit="us per SpMV", cpu=cpu, higher=False,
+                 extra={"bit_exact_vs_oracle": exact, "dominant_kernel_us": round(mean_dom * 1e3, 2),
+                        "gbs_cache_resident": round(algo / (mean_ms * 1e-3) / 1e9, 1)})]
+
+
+def _rmat_all(sp, dt):
+    vals = sp.dist(sp.DIST_UNIFORM, -1, 1) if dt >= 4 else sp.dist(sp.DIST_UNIFORM, -4, 4)
+    return sp.gen_rmat(20, 1 << 24, 0.57, 0.19, 0.19, dtype=dt, vals=vals, seed=2), vals
+
+
+def wl_formats(sp, ctx, args, orc):
+    import numpy as np
+    out = []
+    kinds = ["CSR-row", "CSR-nnz", "COO-row", "COO-nnz-rgrn", "COO-nnz", "BCSR-block", "BCSR-nnz", "BCOO-block",
+             "BCOO-nnz"]
+    for dt in range(6):
+        A, vals = _rmat_all(sp, dt)
+        x = sp.gen_vector(dt, A.ncols, d=vals, seed=22)
+        xd = sp.DeviceArray.from_numpy(ctx, x)
+        yd = sp.DeviceArray(ctx, A.nrows, x.dtype)
+        for kname in kinds:
+            plan = sp.Plan(ctx, kname, A)
+            info = plan.info()
+            mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
+            ok = parity_ok(orc, dt, A, x, yd.numpy())
+            gbs = info["algo_bytes"] / (mean_dom * 1e-3) / 1e9
+            log(f"[bench] formats {DT_NAME[dt]} {kname:13s} {mean_ms * 1e3:8.2f} us  {gbs:7.0f} GB/s  parity {ok}")
+            cfg = {"workload": f"R-MAT 2^20, 2^24 nnz, {kname}, MALL flushed", "kernel": kname,
+                   "nnz": A.nnz, "algo_bytes": info["algo_bytes"]}
+            out.append(line(args, info["algo_bytes"] / (mean_ms * 1e-3) / 1e9, mean_ms, DT_NAME[dt], cfg,
+                            roofline={"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                            extra={"parity_vs_oracle": ok, "launches": info["launches"]}))
+            del plan
+    return out
+
+
+def wl_stencil27(sp, ctx, args, orc):
+    out = []
+    kinds = ["CSR-nnz", "BCSR-block", "DCSR", "DCOO", "RBDCSR", "RBDBCSR-block", "BDCSR", "BDBCOO-block"]
+    for dt in (sp.F32, sp.F64):
+        A = sp.gen_stencil27(64, 4, True, dt, seed=3)
+        x = sp.gen_vector(dt, A.ncols, d=sp.dist(sp.DIST_NORMAL, 0, 1), seed=33)
+        xd = sp.DeviceArray.from_numpy(ctx, x)
+        yd = sp.DeviceArray(ctx, A.nrows, x.dtype)
+        for kname in kinds:
+            t0 = time.time()
+            plan = sp.Plan(ctx, kname, A, sp.opts(vert=args.vert) if kname[0] in "DRB" and kname[1] != "C" else None)
+            info = plan.info()
+            build = time.time() - t0
+            mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
+            ok = parity_ok(orc, dt, A, x, yd.numpy())
+            gbs = info["algo_bytes"] / (mean_dom * 1e-3) / 1e9
+            log(f"[bench] stencil27 {DT_NAME[dt]} {kname:13s} {mean_ms * 1e3:8.2f} us {gbs:7.0f} GB/s "
+                f"(dominant) parity {ok} build {build:.1f}s")
+            cfg = {"workload": f"64^3 x 4 DOF 27-point stencil, periodic, {kname}", "kernel": kname, "nnz": A.nnz,
+                   "algo_bytes": info["algo_bytes"], "vert": info.get("vert", 0)}
+            out.append(line(args, info["algo_bytes"] / (mean_ms * 1e-3) / 1e9, mean_ms, DT_NAME[dt], cfg,
+                            roofline={"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                            extra={"parity_vs_oracle": ok, "launches": info["launches"], "build_s": round(build, 1)}))
+            del plan
+    return out
+
+
+def wl_batched(sp, ctx, args, orc):
+    import numpy as np
+    out = []
+    rng = np.random.default_rng(4)
+    nprob = args.nprob
+    specs = []
+    for i in range(nprob):
+        rows = int(math.exp(rng.uniform(math.log(1000), math.log(200000))))
+        specs.append((i % 3, rows, [1, 4, 8, 32][i % 4], 4000 + i))
+    for dt in (sp.F32, sp.I16):
+        mats, plans, ks = [], [], []
+        tune_s = 0.0
+        for n_done, (kind, rows, k, seed) in enumerate(specs):
+            if n_done % 32 == 0:
+                log(f"[bench] batched {DT_NAME[dt]}: generating and tuning problem {n_done}/{nprob}")
+            if kind == 0:
+                A = sp.gen_uniform(rows, rows, 2, 64, dtype=dt, seed=seed)
+            elif kind == 1:
+                A = sp.gen_banded(rows, 32, 2, 64, dtype=dt, seed=seed)
+            else:
+                scale = max(10, int(round(math.log2(rows))))
+                A = sp.gen_rmat(scale, (1 << scale) * 33, dtype=dt, seed=seed)
+            t0 = time.perf_counter()
+            pts = sp.tune_space(A, sp.KID["CSR-nnz"])
+            sp.tune(ctx, A, pts, iters=3)
+            best = sp.tune_best(pts)
+            tune_s += time.perf_counter() - t0
+            o = sp.point_opts(pts[best]) if best >= 0 else None
+            if k > 1 and o is not None and o.algo == sp.ALGO_SEGMENTED and o.items_per_lane > 8:
+                o.items_per_lane = 8  # the multi-RHS merge kernel takes at most 8 items per lane
+            mats.append(A)
+            plans.append(sp.Plan(ctx, "CSR-nnz", A, o))
+            ks.append(k)
+        Xs, Xd, Yd = [], [], []
+        for i, A in enumerate(mats):
+            if dt >= 4:
+                X = np.random.default_rng(i).standard_normal((A.ncols, ks[i])).astype(np.float32)
+            else:
+                X = np.random.default_rng(i).integers(-4, 5, (A.ncols, ks[i])).astype(np.int16)
+            Xs.append(X)
+            Xd.append(sp.DeviceArray.from_numpy(ctx, X))
+            Yd.append(sp.DeviceArray(ctx, (A.nrows, ks[i]), X.dtype))
+        xs, ys = [x.addr for x in Xd], [y.addr for y in Yd]
+        b = sp.Batch(ctx, plans, ks, graph=True)
+        binfo = b.info()
+        flush = args.flush_mb << 20
+        b.time(xs, ys, 3, flush)
+        tb = statistics.mean(b.time(xs, ys, args.steps, flush))
+        b.time_unbatched(xs, ys, ks, 2, flush)
+        tu = statistics.mean(b.time_unbatched(xs, ys, ks, args.steps, flush))
+        # parity of the batched result against the oracle
+        b.run(xs, ys)
+        ctx.sync()
+        ok = True
+        for i, A in enumerate(mats[:: max(1, nprob // 32)]):
+            j = i * max(1, nprob // 32)
+            Y = Yd[j].numpy()
+            for q in range(ks[j]):
+                ok &= parity_ok(orc, dt, A, np.ascontiguousarray(Xs[j][:, q]), np.ascontiguousarray(Y[:, q]))
+        sv = 4 if dt == sp.F32 else 2
+        algo = sum(p.info()["algo_bytes"] - (A.nrows + A.ncols) * sv + k * (A.nrows + A.ncols) * sv
+                   for p, A, k in zip(plans, mats, ks))
+        nnz = sum(A.nnz for A in mats)
+        gbs_b, gbs_u = algo / (tb * 1e-3) / 1e9, algo / (tu * 1e-3) / 1e9
+        log(f"[bench] batched {DT_NAME[dt]}: {nprob} problems, {nnz} nnz, graph {tb:.3f} ms ({gbs_b:.0f} GB/s), "
+            f"per-problem launches {tu:.3f} ms ({gbs_u:.0f} GB/s), groups {binfo['groups']}, parity {ok}")
+        cfg = {"workload": f"{nprob} problems (uniform / banded bw 32 / R-MAT), rows log-U[1k,200k], "
+                           f"nnz/row 2..64, k in {{1,4,8,32}}, one persistent hipGraph launch per kind group",
+               "problems": nprob, "nnz_total": nnz, "algo_bytes": algo, "groups": binfo["groups"]}
+        out.append(line(args, gbs_b, tb, DT_NAME[dt], cfg,
+                        roofline={"bound": "hbm", "achieved": round(gbs_b, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs_b / HBM_PEAK_GBS, 4), "traffic": None},
+                        extra={"per_problem_launch_ms": round(tu, 4), "per_problem_gbs": round(gbs_u, 1),
+                               "speedup_vs_per_problem": round(tu / tb, 3), "parity_vs_oracle": bool(ok),
+                               "autotune_s": round(tune_s, 2)}))
+    return out
+
+
+def wl_selector(sp, ctx, args, orc):
+    import numpy as np
+    from sparsep_amd import selector as S
+    model = os.path.join(ROOT, "sparsep_amd", "data", "selector_f32.bin")
+    sel = sp.Selector(ctx, path=model)
+    specs = S.corpus_specs(args.sel_count * 5, seed=args.sel_seed, rows=(10_000, 2_000_000), nnz=(50_000, 8_000_000))
+    specs = specs[4::5][:args.sel_count]  # every fifth spec of an unseen seed
+    ratios, sel_ms, tune_ms = [], [], []
+    for s in specs:
+        A = S.build_matrix(s)
+        t0 = time.perf_counter()
+        pt, scores, _ = sel.select(A, fallback_k=0)
+        sel_ms.append((time.perf_counter() - t0) * 1e3)
+        t0 = time.perf_counter()
+        pts = sp.tune(ctx, A, iters=5, flush_bytes=512 << 20)
+        tune_ms.append((time.perf_counter() - t0) * 1e3)
+        ok = [p.ms for p in pts if p.status == 0 and p.ms > 0]
+        # the pick runs its default configuration: the first point of its id
+        first = next((p for p in pts if p.kernel_id == pt.kernel_id), None)
+        good = first is not None and first.status == 0 and first.ms > 0
+        ratios.append(min(ok) / first.ms if good else 0.0)
+        log(f"[bench] selector {s['family']} nnz={A.nnz} pick {sp.KERNELS[pt.kernel_id]} ratio {ratios[-1]:.3f}")
+    r = np.array(ratios)
+    geo = float(np.exp(np.mean(np.log(np.maximum(r, 1e-9)))))
+    cfg = {"workload": f"learned variant selector on {len(specs)} unseen synthetic matrices (seed {args.sel_seed}), "
+                       f"FP32, rows 10k-2M, nnz 50k-8M; pick = default configuration of the predicted id, "
+                       f"best = live exhaustive autotuner", "matrices": len(specs)}
+    return [line(args, geo, statistics.mean(sel_ms), "f32", cfg, unit="learned-pick / exhaustive-best throughput",
+                 extra={"within_10pct": float(np.mean(r >= 1 / 1.1)), "min_ratio": float(r.min()),
+                        "select_ms_median": float(np.median(sel_ms)), "tune_ms_median": float(np.median(tune_ms)),
+                        "overhead_median": float(np.median(np.array(sel_ms) / np.array(tune_ms)))})]
+
+
+WORKLOADS = {"rmat": wl_rmat, "stencil5": wl_stencil5, "formats": wl_formats, "stencil27": wl_stencil27,
+             "batched": wl_batched, "selector": wl_selector}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="rmat", choices=sorted(WORKLOADS))
+    ap.add_argument("--flush-mb", type=int, default=1024)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline timing")
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--nnz-log2", type=int, default=24)
+    ap.add_argument("--vert", type=int, default=0, help="stencil27: vertical partitions of the 2D kernels")
+    ap.add_argument("--nprob", type=int, default=256, help="batched: number of problems")
+    ap.add_argument("--sel-count", type=int, default=40, help="selector: matrices evaluated")
+    ap.add_argument("--sel-seed", type=int, default=55)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build targets a single MI355X and has no "
+              f"multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    import sparsep_amd.api as sp
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+
+    ctx = sp.Context(0)
+    info = ctx.info()
+    log(f"[bench] device {info['arch']} CUs={info['num_cus']} HBM={info['hbm_bytes'] / 2**30:.0f} GiB "
+        f"workload {args.workload}")
+    for rec in WORKLOADS[args.workload](sp, ctx, args, orc):
+        print(json.dumps(rec), flush=True)
 
 
 if __name__ == "__main__":

@@ -503,6 +503,11 @@ int sp_batch_destroy(sp_batch *b);
 /* create + run + destroy */
 int sp_batch_run_once(sp_ctx *ctx, int nprob, sp_plan *const *plans, const void *const *X,
                       void *const *Y, const int *k);
+/* The unbatched counterpart, for comparison: one sp_spmm per problem back
+ * to back on the ctx stream, `iters` times; ms[i] = HIP-event time of the
+ * whole sequence (L2/MALL flushed with flush_bytes before each). */
+int sp_time_plans(sp_ctx *ctx, int nprob, sp_plan *const *plans, const void *const *X,
+                  void *const *Y, const int *k, int iters, uint64_t flush_bytes, float *ms);
 
 /* Exhaustive autotuner (no counterpart in the reference).  A point is one
  * kernel id with one option set; sp_tune_space enumerates the candidate

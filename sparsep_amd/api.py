@@ -526,6 +526,16 @@ class Batch:
         check(lib().sp_batch_time(self.ptr, self._tab(xs), self._tab(ys), iters, flush_bytes, ms), "sp_batch_time")
         return list(ms)
 
+    def time_unbatched(self, xs, ys, k, iters, flush_bytes=0):
+        """The same problems as one sp_spmm launch sequence per problem."""
+        n = len(self.plans)
+        arr = (ctypes.c_void_p * n)(*[p.ptr.value for p in self.plans])
+        ks = (ctypes.c_int * n)(*[int(v) for v in k])
+        ms = (ctypes.c_float * iters)()
+        check(lib().sp_time_plans(self.ctx.ptr, n, arr, self._tab(xs), self._tab(ys), ks, iters, flush_bytes, ms),
+              "sp_time_plans")
+        return list(ms)
+
     def info(self):
         ng, ab = ctypes.c_uint32(), ctypes.c_uint64()
         check(lib().sp_batch_info(self.ptr, ctypes.byref(ng), ctypes.byref(ab)))

@@ -311,3 +311,32 @@ extern "C" int sp_batch_run_once(sp_ctx *ctx, int nprob, sp_plan *const *plans, 
     const int rc2 = sp_batch_destroy(b);
     return rc ? rc : rc2;
 }
+
+// The unbatched counterpart of sp_batch_time: one sp_spmm call per problem,
+// back to back on the context stream, timed as a whole with HIP events.
+extern "C" int sp_time_plans(sp_ctx *ctx, int nprob, sp_plan *const *plans, const void *const *X, void *const *Y,
+                             const int *k, int iters, uint64_t flush_bytes, float *ms) {
+    if (!ctx || nprob <= 0 || !plans || !X || !Y || !k || iters <= 0 || !ms) return SP_ERR_ARG;
+    for (int i = 0; i < nprob; i++)
+        if (!plans[i] || plans[i]->ctx != ctx || k[i] <= 0) return SP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (flush_bytes && ctx->flush_bytes < flush_bytes) {
+        if (ctx->flush_buf) (void)hipFree(ctx->flush_buf);
+        ctx->flush_buf = nullptr;
+        ctx->flush_bytes = 0;
+        if (hipMalloc(&ctx->flush_buf, flush_bytes) != hipSuccess) return SP_ERR_NOMEM;
+        ctx->flush_bytes = flush_bytes;
+    }
+    int rc;
+    for (int it = 0; it < iters; it++) {
+        if (flush_bytes && (rc = launch_flush(ctx->stream, ctx->flush_buf, flush_bytes, 0x51ED2701u * (uint32_t)(it + 1))))
+            return rc;
+        SP_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        for (int i = 0; i < nprob; i++)
+            if ((rc = sp_spmm(plans[i], k[i], X[i], k[i], Y[i], k[i]))) return rc;
+        SP_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        SP_HIP_TRY(hipEventSynchronize(ctx->ev1));
+        SP_HIP_TRY(hipEventElapsedTime(&ms[it], ctx->ev0, ctx->ev1));
+    }
+    return SP_OK;
+}
