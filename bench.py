@@ -211,6 +211,32 @@ def wl_stencil27(sp, ctx, args, orc):
     return out
 
 
+def wl_rmat2d(sp, ctx, args, orc):
+    """The R-MAT FP32 matrix under the 2D tilings, over a range of vertical splits."""
+    import numpy as np
+    out = []
+    A, vals = _rmat_all(sp, sp.F32)
+    x = sp.gen_vector(sp.F32, A.ncols, d=vals, seed=22)
+    xd = sp.DeviceArray.from_numpy(ctx, x)
+    yd = sp.DeviceArray(ctx, A.nrows, np.float32)
+    for kname in ("RBDCSR", "BDCSR", "BDCOO", "DCSR"):
+        for vert in (2, 4, 8, 16, 32):
+            plan = sp.Plan(ctx, kname, A, sp.opts(vert=vert))
+            info = plan.info()
+            mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
+            ok = parity_ok(orc, sp.F32, A, x, yd.numpy())
+            csr_bytes = 146800644
+            log(f"[bench] rmat2d {kname:8s} vert {vert:2d} {mean_ms * 1e3:8.2f} us (dominant {mean_dom * 1e3:8.2f}) "
+                f"CSR-equivalent {csr_bytes / (mean_ms * 1e-3) / 1e9:6.0f} GB/s parity {ok}")
+            cfg = {"workload": f"R-MAT 2^20 FP32, {kname} vert {vert}", "kernel": kname, "vert": vert,
+                   "algo_bytes": info["algo_bytes"]}
+            out.append(line(args, info["algo_bytes"] / (mean_ms * 1e-3) / 1e9, mean_ms, "f32", cfg,
+                            extra={"parity_vs_oracle": ok, "launches": info["launches"],
+                                   "csr_equivalent_gbs": round(csr_bytes / (mean_ms * 1e-3) / 1e9, 1)}))
+            del plan
+    return out
+
+
 def wl_batched(sp, ctx, args, orc):
     import numpy as np
     out = []
@@ -323,6 +349,7 @@ def wl_selector(sp, ctx, args, orc):
 
 
 WORKLOADS = {"rmat": wl_rmat, "stencil5": wl_stencil5, "formats": wl_formats, "stencil27": wl_stencil27,
+             "rmat2d": wl_rmat2d,
              "batched": wl_batched, "selector": wl_selector}
 
 

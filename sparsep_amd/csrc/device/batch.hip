@@ -117,11 +117,11 @@ extern "C" int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, co
         const sp_plan *p = plans[i];
         const Work &w = p->work;
         b->algo_bytes += p->algo_bytes + (uint64_t)(k[i] - 1) * ((uint64_t)p->nrows + p->ncols) * sp_dtype_size(dt);
-        const int kmax = w.kind == WK_BCOO_SEG ? 1 : 8;
+        const int kmax = max_rhs_chunk(w);
         for (int done = 0; done < k[i];) {
             const int left = k[i] - done;
-            int kc = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
-            kc = std::min(kc, kmax);
+            int kc = left >= 32 ? 32 : left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+            while (kc > kmax) kc >>= 1;
             const auto key = std::make_tuple(w.kind, w.ipt, w.lanes, w.R, w.C, kc);
             auto it = gidx.find(key);
             size_t gi;

@@ -3,6 +3,8 @@
 // kind needs one.  Included only by the per-dtype instantiation files.
 #pragma once
 
+#include <algorithm>
+
 #include "plan.hpp"
 #include "work.hpp"
 
@@ -13,6 +15,29 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
     using A = acc_t<T>;
     const dim3 grid(w.grid_cap && w.grid_cap < w.nunits ? w.grid_cap : w.nunits), block(TPB);
     A *cv = (A *)w.d_carry_val;
+    if constexpr (K > 8) {
+        // wide chunks: row-granular kernels only (no carries, no LDS staging)
+        if (w.kind == WK_CSR_VEC) {
+#define SPD_VECW(Lv) case Lv: hipLaunchKernelGGL((csr_vector_kernel<T, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
+            switch (w.lanes) {
+                SPD_VECW(1) SPD_VECW(2) SPD_VECW(4) SPD_VECW(8) SPD_VECW(16) SPD_VECW(32) SPD_VECW(64)
+            default: return SP_ERR_ARG;
+            }
+#undef SPD_VECW
+            return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+        }
+        if (w.kind == WK_BCSR_VEC) {
+#define SPD_BLW(Rv, Cv, Lv) \
+    if (w.R == Rv && w.C == Cv && w.lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP; }
+#define SPD_BRCW(Rv, Cv) SPD_BLW(Rv, Cv, 1) SPD_BLW(Rv, Cv, 4) SPD_BLW(Rv, Cv, 16) SPD_BLW(Rv, Cv, 64)
+            if constexpr (K <= 32) { SPD_BRCW(1, 1) SPD_BRCW(2, 2) SPD_BRCW(2, 4) }
+            if constexpr (K <= 16) { SPD_BRCW(4, 4) SPD_BRCW(4, 2) }
+#undef SPD_BRCW
+#undef SPD_BLW
+            return SP_ERR_UNSUPPORTED;
+        }
+        return SP_ERR_UNSUPPORTED;
+    } else {
     switch (w.kind) {
     case WK_CSR_MERGE:
         if (K > 1 && w.ipt > 8) return SP_ERR_UNSUPPORTED;
@@ -66,6 +91,7 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
     default: return SP_ERR_ARG;
     }
     return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+    }
 }
 
 template <typename T, int K>
@@ -106,6 +132,8 @@ int run_tables_t(const Work &w, sp_ctx *ctx, const uintptr_t *xtab, const uintpt
     io.ldx = io.ldy = 0;
     io.q0 = 0;
     switch (kc) {
+    case 32: return launch_chunk<T, 32>(w, ctx, io);
+    case 16: return launch_chunk<T, 16>(w, ctx, io);
     case 8: return launch_chunk<T, 8>(w, ctx, io);
     case 4: return launch_chunk<T, 4>(w, ctx, io);
     case 2: return launch_chunk<T, 2>(w, ctx, io);
@@ -119,8 +147,7 @@ int run_work_t(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintp
                const uintptr_t *ytab, int k, int ldx, int ldy) {
     using namespace spd;
     int done = 0;
-    // Block COO kernels take one right-hand side per launch.
-    const int kmax = (w.kind == WK_BCOO_SEG) ? 1 : 8;
+    const int kmax = max_rhs_chunk(w);
     if (w.zero_all && w.zero_rows && yb) {
         // Partial buffers of tiled plans: rows a tile has no entries for stay 0.
         int rc = launch_fill_zero(ctx->stream, yb, (uint64_t)w.zero_rows * ldy * sizeof(T));
@@ -128,8 +155,8 @@ int run_work_t(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintp
     }
     while (done < k) {
         const int left = k - done;
-        int kc = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
-        if (kc > kmax) kc = kmax;
+        int kc = left >= 32 ? 32 : left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+        while (kc > kmax) kc >>= 1;
         if ((w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG) && (uint32_t)kc > w.carry_k) kc = (int)w.carry_k;
         Ptrs<T> io;
         io.xb = xb ? (const T *)xb + done : nullptr;
@@ -139,7 +166,9 @@ int run_work_t(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintp
         io.ldx = ldx;
         io.ldy = ldy;
         io.q0 = done;
-        int rc = kc == 8 ? launch_chunk<T, 8>(w, ctx, io)
+        int rc = kc == 32 ? launch_chunk<T, 32>(w, ctx, io)
+               : kc == 16 ? launch_chunk<T, 16>(w, ctx, io)
+               : kc == 8 ? launch_chunk<T, 8>(w, ctx, io)
                : kc == 4 ? launch_chunk<T, 4>(w, ctx, io)
                : kc == 2 ? launch_chunk<T, 2>(w, ctx, io)
                          : launch_chunk<T, 1>(w, ctx, io);

@@ -48,7 +48,7 @@ static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
             v.push_back(p);
         }
     } else if (id >= SP_K_COO_ROW && id <= SP_K_COO_NNZ) {
-        for (uint32_t I : {4u, 16u}) {
+        for (uint32_t I : {8u, 16u}) {
             Pt p{id, base};
             p.o.items_per_lane = I;
             v.push_back(p);

@@ -31,7 +31,19 @@ struct Work {
     uint32_t grid_cap = 0;  // persistent launch: at most this many workgroups (0 = one per unit)
 };
 
-// Launches `w` for k right-hand sides (processed in chunks of 8/4/2/1).
+// Widest right-hand-side chunk a kernel kind takes in one pass: the
+// row-granular kernels keep K (CSR) or R*K (BCSR, R*K <= 64) accumulators per
+// lane; the kernels with carries keep K partial sums per carry slot.
+inline int max_rhs_chunk(const Work &w) {
+    switch (w.kind) {
+    case WK_CSR_VEC: return 32;
+    case WK_BCSR_VEC: return w.R <= 2 ? 32 : w.R <= 4 ? 16 : 8;
+    case WK_BCOO_SEG: return 1;
+    default: return 8;
+    }
+}
+
+// Launches `w` for k right-hand sides (processed in chunks of 32/16/8/4/2/1).
 // x/y: per-call bases (element offsets from the descriptors apply), or
 // xtab/ytab: per-problem device pointer tables.
 int run_work(const Work &w, sp_ctx *ctx, const void *xb, void *yb, const uintptr_t *xtab,

@@ -1,6 +1,10 @@
 /* Structural features of a sparse matrix for the learned variant selector.
- * Everything is O(nnz + n + m) host work over one counting sort by row, so
- * extraction costs a small fraction of building even one plan.  The block
+ * Row and column lengths come from one pass over all entries; the
+ * per-entry structure (bandwidth, diagonal distances, locality, block fill)
+ * from up to 32 evenly spaced runs of at most 512 rows and about 32K
+ * entries each (all rows when there are at most 16,384).  Input sorted by row is read in place, otherwise grouped by
+ * a counting sort first.  Extraction costs a small fraction of building one
+ * plan.  The block
  * fill ratios restate the occupancy the reference computes when it builds
  * BCOO (spmv/1D/BCOO-block/support/matrix.h:491-505), for 2x2 and 4x4
  * blocks, without building the blocks.  Feature layout (index: meaning):
@@ -70,27 +74,54 @@ int sp_features(const sp_coo *A, float *f) {
     uint32_t *rlen = calloc(n + 1, sizeof(uint32_t));
     uint32_t *clen = calloc(m, sizeof(uint32_t));
     uint64_t *rptr = malloc((n + 1) * sizeof(uint64_t));
-    uint32_t *cols = malloc(nnz * sizeof(uint32_t));
-    uint32_t *mark2 = malloc(((m + 1) / 2 + 1) * sizeof(uint32_t));
-    uint32_t *mark4 = malloc(((m + 3) / 4 + 1) * sizeof(uint32_t));
+    uint32_t *owned = NULL;
+    const uint32_t *cols = A->colind;
     int rc = SP_ERR_NOMEM;
     uint64_t *rh = NULL, *ch = NULL;
-    if (!rlen || !clen || !rptr || !cols || !mark2 || !mark4) goto done;
-    for (uint64_t k = 0; k < nnz; k++) {
-        const uint32_t r = A->rowind[k], c = A->colind[k];
-        if (r >= n || c >= m) { rc = SP_ERR_ARG; goto done; }
-        rlen[r]++;
-        clen[c]++;
+    if (!rlen || !clen || !rptr) goto done;
+    int sorted = 1;
+    {
+        /* counts with relaxed atomic increments over host threads (the
+         * passes are memory bound; the result does not depend on order) */
+        const uint32_t *ri = A->rowind, *ci = A->colind;
+        int bad = 0;
+#pragma omp parallel reduction(| : bad) reduction(& : sorted) if (nnz > (1u << 18))
+        {
+            /* row counts: runs of equal rows are counted locally and added once */
+            uint32_t cur = UINT32_MAX, run = 0;
+#pragma omp for schedule(static) nowait
+            for (uint64_t k = 0; k < nnz; k++) {
+                const uint32_t r = ri[k];
+                bad |= (r >= n);
+                sorted &= (k == 0 || ri[k - 1] <= r);
+                if (r != cur) {
+                    if (run) __atomic_fetch_add(&rlen[cur], run, __ATOMIC_RELAXED);
+                    cur = r < n ? r : (uint32_t)n;
+                    run = 0;
+                }
+                run++;
+            }
+            if (run) __atomic_fetch_add(&rlen[cur], run, __ATOMIC_RELAXED);
+#pragma omp for schedule(static)
+            for (uint64_t k = 0; k < nnz; k++) {
+                const uint32_t c = ci[k];
+                bad |= (c >= m);
+                __atomic_fetch_add(&clen[c < m ? c : 0], 1u, __ATOMIC_RELAXED);
+            }
+        }
+        if (bad) { rc = SP_ERR_ARG; goto done; }
     }
     rptr[0] = 0;
     for (uint64_t r = 0; r < n; r++) rptr[r + 1] = rptr[r] + rlen[r];
-    {
+    if (!sorted) {
         /* stable counting sort by row (input order kept inside a row) */
         uint64_t *pos = malloc(n * sizeof(uint64_t));
-        if (!pos) goto done;
+        owned = malloc(nnz * sizeof(uint32_t));
+        if (!pos || !owned) { free(pos); goto done; }
         memcpy(pos, rptr, n * sizeof(uint64_t));
-        for (uint64_t k = 0; k < nnz; k++) cols[pos[A->rowind[k]]++] = A->colind[k];
+        for (uint64_t k = 0; k < nnz; k++) owned[pos[A->rowind[k]]++] = A->colind[k];
         free(pos);
+        cols = owned;
     }
 
     /* row-length statistics */
@@ -168,55 +199,98 @@ int sp_features(const sp_coo *A, float *f) {
     for (uint64_t c = 0; c < m; c++) ch[clen[c]]++;
     f[34] = gini_hist(ch, cmax, m, nnz);
 
-    /* bandwidth, locality and block fill in one pass over the rows */
+    /* bandwidth, locality and block fill over the sampled row runs */
     {
         double dsum = 0, dmax = 0;
-        uint64_t near = 0, runs = 0, close = 0, lines = 0, nb2 = 0, nb4 = 0;
+        uint64_t near = 0, runs = 0, close = 0, lines = 0, nb2 = 0, nb4 = 0, sent = 0, srows = 0;
         uint64_t dh[8] = {0};
         double jump = 0;
-        memset(mark2, 0xFF, ((m + 1) / 2 + 1) * sizeof(uint32_t));
-        memset(mark4, 0xFF, ((m + 3) / 4 + 1) * sizeof(uint32_t));
+        int nomem = 0;
+        for (uint64_t r = 1; r < n; r++) jump += fabs((double)rlen[r] - (double)rlen[r - 1]);
         const double slope = (double)m / (double)n;
-        for (uint64_t r = 0; r < n; r++) {
-            const int64_t dg = (int64_t)floor((double)r * slope);
-            const uint32_t b2 = (uint32_t)(r / 2), b4 = (uint32_t)(r / 4);
-            int64_t prev = -1000;
-            if (r) jump += fabs((double)rlen[r] - (double)rlen[r - 1]);
-            for (uint64_t k = rptr[r]; k < rptr[r + 1]; k++) {
-                const int64_t c = cols[k];
-                const int64_t d = c - dg;
-                const double ad = (double)(d < 0 ? -d : d);
-                dsum += ad;
-                if (ad > dmax) dmax = ad;
-                near += (ad <= 32);
-                int bin;
-                if (ad == 0) bin = 0;
-                else if (ad < 2) bin = 1;
-                else if (ad < 4) bin = 2;
-                else if (ad < 16) bin = 3;
-                else if (ad < 256) bin = 4;
-                else if (ad < 4096) bin = 5;
-                else if (ad < 65536) bin = 6;
-                else bin = 7;
-                dh[bin]++;
-                runs += (c == prev + 1);
-                close += (c - prev < 16 && prev - c < 16);
-                lines += (prev < 0 || (c >> 4) != (prev >> 4));
-                prev = c;
-                if (mark2[c / 2] != b2) { mark2[c / 2] = b2; nb2++; }
-                if (mark4[c / 4] != b4) { mark4[c / 4] = b4; nb4++; }
+        const uint64_t RUN = 512, NRUN = 32;
+        const uint64_t nruns = n <= RUN * NRUN ? 1 : NRUN;
+        /* bins: 0 | 1 | 2-3 | 4-15 | 16-255 | 256-4095 | 4096-65535 | more */
+        static const uint8_t bin_of_log2[64] = {1, 2, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 6, 6, 6, 6,
+                                                7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7,
+                                                7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7,
+                                                7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7};
+#pragma omp parallel reduction(+ : dsum, near, runs, close, lines, nb2, nb4, sent, srows, dh[:8]) \
+    reduction(max : dmax) reduction(| : nomem) if (nruns > 1)
+        {
+            uint64_t *set = NULL;
+            uint64_t cap = 0;
+#pragma omp for schedule(dynamic)
+            for (uint64_t q = 0; q < nruns; q++) {
+                const uint64_t r0 = nruns == 1 ? 0 : ((q * (n - RUN)) / (NRUN - 1)) & ~(uint64_t)3;
+                uint64_t r1 = nruns == 1 ? n : r0 + RUN;
+                /* a run stops after ~32K entries (whole 4-row groups) so hub rows
+                 * do not dominate the cost */
+                if (nruns > 1)
+                    while (r1 > r0 + 4 && rptr[r1 - 4] - rptr[r0] >= (1u << 15)) r1 -= 4;
+                const uint64_t ne = rptr[r1] - rptr[r0];
+                /* distinct (block row, block column) keys of the run: open addressing,
+                 * 2x2 keys tagged with bit 63 to share one table with the 4x4 keys */
+                uint64_t want = 16;
+                while (want < 4 * ne) want <<= 1;
+                if (want > cap) {
+                    free(set);
+                    set = malloc(want * sizeof(uint64_t));
+                    cap = set ? want : 0;
+                    if (!set) { nomem = 1; continue; }
+                }
+                memset(set, 0xFF, want * sizeof(uint64_t));
+                const uint64_t mask = want - 1;
+                srows += r1 - r0;
+                sent += ne;
+                for (uint64_t r = r0; r < r1; r++) {
+                    const int64_t dg = (int64_t)floor((double)r * slope);
+                    int64_t prev = -1000;
+                    for (uint64_t k = rptr[r]; k < rptr[r + 1]; k++) {
+                        const int64_t c = cols[k];
+                        const int64_t d = c - dg;
+                        const uint64_t ud = (uint64_t)(d < 0 ? -d : d);
+                        dsum += (double)ud;
+                        if ((double)ud > dmax) dmax = (double)ud;
+                        near += (ud <= 32);
+                        dh[ud ? bin_of_log2[63 - __builtin_clzll(ud)] : 0]++;
+                        runs += (c == prev + 1);
+                        close += (c - prev < 16 && prev - c < 16);
+                        lines += (prev < 0 || (c >> 4) != (prev >> 4));
+                        prev = c;
+                        const uint64_t key2 = (1ULL << 63) | ((uint64_t)(r / 2) << 32) | (uint64_t)(c / 2);
+                        const uint64_t key4 = ((uint64_t)(r / 4) << 32) | (uint64_t)(c / 4);
+                        for (int z = 0; z < 2; z++) {
+                            const uint64_t key = z ? key4 : key2;
+                            uint64_t h = (key * 0x9E3779B97F4A7C15ULL) >> 20;
+                            for (;; h++) {
+                                uint64_t *slot = &set[h & mask];
+                                if (*slot == key) break;
+                                if (*slot == UINT64_MAX) {
+                                    *slot = key;
+                                    if (z) nb4++;
+                                    else nb2++;
+                                    break;
+                                }
+                            }
+                        }
+                    }
+                }
             }
+            free(set);
         }
-        f[35] = (float)(dsum / (double)nnz / (double)m);
+        if (nomem) goto done;
+        const double se = sent ? (double)sent : 1.0;
+        f[35] = (float)(dsum / se / (double)m);
         f[36] = (float)(dmax / (double)m);
-        f[37] = (float)((double)near / (double)nnz);
-        for (int b = 0; b < 8; b++) f[38 + b] = (float)((double)dh[b] / (double)nnz);
-        f[46] = (float)((double)nnz / (4.0 * (double)nb2));
-        f[47] = (float)((double)nnz / (16.0 * (double)nb4));
-        f[48] = lg2((double)nb4 / (double)((n + 3) / 4)) / 12.f;
-        f[49] = (float)((double)runs / (double)nnz);
-        f[50] = (float)((double)close / (double)nnz);
-        f[51] = (float)((double)lines / (double)nnz);
+        f[37] = (float)((double)near / se);
+        for (int b = 0; b < 8; b++) f[38 + b] = (float)((double)dh[b] / se);
+        f[46] = nb2 ? (float)((double)sent / (4.0 * (double)nb2)) : 0.f;
+        f[47] = nb4 ? (float)((double)sent / (16.0 * (double)nb4)) : 0.f;
+        f[48] = lg2((double)nb4 / (double)((srows + 3) / 4)) / 12.f;
+        f[49] = (float)((double)runs / se);
+        f[50] = (float)((double)close / se);
+        f[51] = (float)((double)lines / se);
         f[52] = (float)(jump / (double)n / (ravg + 1.0));
     }
     rc = SP_OK;
@@ -224,9 +298,7 @@ done:
     free(rlen);
     free(clen);
     free(rptr);
-    free(cols);
-    free(mark2);
-    free(mark4);
+    free(owned);
     free(rh);
     free(ch);
     return rc;

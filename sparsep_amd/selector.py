@@ -155,6 +155,28 @@ def point_cost_ms(rec, ids=None):
 # --------------------------------------------------------------------------
 # training and evaluation
 # --------------------------------------------------------------------------
+def read_model(path):
+    """Fields of a model file written by sp_selector_save: sizes[4],
+    margin, feature mean[64], feature std[64], parameters."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    if raw[:8] != b"SPSEL001":
+        raise ValueError(f"{path}: not a selector model file")
+    sizes = tuple(int(v) for v in np.frombuffer(raw[8:24], np.int32))
+    margin = float(np.frombuffer(raw[24:28], np.float32)[0])
+    mean = np.frombuffer(raw[28:284], np.float32).copy()
+    std = np.frombuffer(raw[284:540], np.float32).copy()
+    params = np.frombuffer(raw[540:], np.float32).copy()
+    return {"sizes": sizes, "margin": margin, "mean": mean, "std": std, "params": params}
+
+
+def model_scores(ctx, model, X):
+    """Scores of feature rows X with the model's MLP (HIP kernels)."""
+    m = api.Mlp(ctx, model["sizes"])
+    m.set_params(model["params"])
+    return m.predict((X - model["mean"]) / np.maximum(model["std"], 1e-6))
+
+
 def standardise(X):
     mean = X.mean(0)
     std = X.std(0)
