@@ -81,8 +81,15 @@ def wl_rmat(sp, ctx, args, orc):
     A = sp.gen_rmat(args.scale, 1 << args.nnz_log2, 0.57, 0.19, 0.19, dtype=dt, seed=2)
     x = sp.gen_vector(dt, A.ncols, seed=22)
     log(f"[bench] R-MAT 2^{args.scale}: nnz={A.nnz} generated in {time.time() - t0:.1f}s")
-    plan = sp.Plan(ctx, "CSR-nnz", A, sp.opts(algo=sp.ALGO_SEGMENTED, items_per_lane=8))
+    # the library's own autotuner picks the CSR-nnz configuration (outside the timed region)
+    pts = sp.tune(ctx, A, sp.tune_space(A, sp.KID["CSR-nnz"]), iters=3, flush_bytes=args.flush_mb << 20)
+    best = sp.tune_best(pts)
+    o = sp.point_opts(pts[best]) if best >= 0 else sp.opts(algo=sp.ALGO_SEGMENTED, items_per_lane=8)
+    plan = sp.Plan(ctx, "CSR-nnz", A, o)
     pinfo = plan.info()
+    algo_name = (f"merge-path, {pinfo['items_per_lane']} items per lane" if pinfo["algo"] == sp.ALGO_SEGMENTED
+                 else f"vector, {pinfo['lanes_per_row']} lanes per row")
+    log(f"[bench] tuned CSR-nnz configuration: {algo_name}")
     algo_bytes = pinfo["algo_bytes"]
     xd = sp.DeviceArray.from_numpy(ctx, x)
     yd = sp.DeviceArray(ctx, A.nrows, np.float32)
@@ -113,12 +120,23 @@ def wl_rmat(sp, ctx, args, orc):
                "host_cpus": orc.num_cpus()}
         log(f"[bench] cpu baseline {cms_cpu:.1f} ms/run ({cpu_gbs:.2f} GB/s, {iters} runs)")
     cfg = {"workload": f"R-MAT 2^{args.scale} x 2^{args.scale}, {A.nnz} unique nnz, a=0.57 b=0.19 c=0.19, "
-                       f"FP32 CSR SpMV (CSR-nnz, merge-path kernel), MALL flushed before each step",
+                       f"FP32 CSR SpMV (CSR-nnz, {algo_name}, chosen by the autotuner), "
+                       f"MALL flushed before each step",
            "nrows": A.nrows, "nnz": A.nnz, "algo_bytes": algo_bytes,
            "gflops": round(2 * A.nnz / (mean_ms * 1e-3) / 1e9, 2)}
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
-    extra = {"frac_of_peak_whole_spmv": round(value / HBM_PEAK_GBS, 4), "measured_copy_gbs": round(copy_gbs, 1),
+    report = os.path.join(ROOT, "sparsep_amd", "data", "selector_report.json")
+    sel = None
+    if os.path.exists(report):
+        with open(report) as f:
+            ho = json.load(f)["held_out"]
+        sel = {"source": "sparsep_amd/data/selector_report.json (held-out corpus, measured when trained)",
+               "learned_pick_over_exhaustive_best_geomean": ho["learned"]["geomean_ratio"],
+               "with_fallback_geomean": ho["learned_with_fallback"]["geomean_ratio"],
+               "selection_overhead_median": ho["overhead"]["select_over_exhaustive_median"]}
+    extra = {"selector": sel,
+             "frac_of_peak_whole_spmv": round(value / HBM_PEAK_GBS, 4), "measured_copy_gbs": round(copy_gbs, 1),
              "frac_of_measured_copy": round(value / copy_gbs, 4), "warm_cache_gbs": round(warm_gbs, 1),
              "kernels_per_spmv": pinfo["launches"], "parity_vs_oracle": parity}
     return [line(args, value, mean_ms, "f32", cfg, roofline=roof, cpu=cpu, extra=extra,

@@ -9,6 +9,7 @@ log2(t[i,k] / best[i]) (clipped to LABEL_CAP; ids whose plan could not be
 built get LABEL_CAP), so the lowest output is the predicted fastest id.
 No counterpart in the reference.
 """
+import gzip
 import json
 import math
 import time
@@ -119,7 +120,9 @@ def tune_corpus(ctx, specs, out_path, iters=5, flush_bytes=512 << 20, start=0, l
 
 
 def load_records(path):
-    with open(path) as f:
+    """Timing records of a corpus file (JSON lines, optionally gzip'd)."""
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "rt") as f:
         return [json.loads(line) for line in f if line.strip()]
 
 

@@ -17,6 +17,11 @@ MODEL = os.path.join(DATA, "selector_f32.bin")
 CORPUS = os.path.join(DATA, "selector_corpus.jsonl.gz")
 
 
+if not (os.path.exists(MODEL) and os.path.exists(CORPUS)):
+    pytest.skip("selector model / corpus not trained yet (python -m sparsep_amd.selector tune|train)",
+                allow_module_level=True)
+
+
 @pytest.fixture(scope="module")
 def heldout(sp, ctx):
     from sparsep_amd import selector as S
