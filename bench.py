@@ -179,7 +179,7 @@ def wl_formats(sp, ctx, args, orc):
     out = []
     kinds = ["CSR-row", "CSR-nnz", "COO-row", "COO-nnz-rgrn", "COO-nnz", "BCSR-block", "BCSR-nnz", "BCOO-block",
              "BCOO-nnz"]
-    for dt in range(6):
+    for dt in args.dtypes:
         A, vals = _rmat_all(sp, dt)
         x = sp.gen_vector(dt, A.ncols, d=vals, seed=22)
         xd = sp.DeviceArray.from_numpy(ctx, x)
@@ -204,7 +204,7 @@ def wl_formats(sp, ctx, args, orc):
 def wl_stencil27(sp, ctx, args, orc):
     out = []
     kinds = ["CSR-nnz", "BCSR-block", "DCSR", "DCOO", "RBDCSR", "RBDBCSR-block", "BDCSR", "BDBCOO-block"]
-    for dt in (sp.F32, sp.F64):
+    for dt in [d for d in args.dtypes if d in (sp.F32, sp.F64)]:
         A = sp.gen_stencil27(64, 4, True, dt, seed=3)
         x = sp.gen_vector(dt, A.ncols, d=sp.dist(sp.DIST_NORMAL, 0, 1), seed=33)
         xd = sp.DeviceArray.from_numpy(ctx, x)
@@ -253,6 +253,52 @@ def wl_rmat2d(sp, ctx, args, orc):
                                    "csr_equivalent_gbs": round(csr_bytes / (mean_ms * 1e-3) / 1e9, 1)}))
             del plan
     return out
+
+
+def wl_phases(sp, ctx, args, orc):
+    """SparseP's five-phase report (spmv/1D/CSR-row/host/app.c:291-300) for CSR-row on the R-MAT
+    FP32 matrix, or on --mtx: host-to-device matrix + partition, x upload, kernel, y download,
+    merge (on the GPU inside the kernel phase here).  PCIe-inclusive, host wall clock."""
+    import numpy as np
+    dt = sp.F32
+    if args.mtx:
+        A = sp.read_mtx(args.mtx, dt, sp.MTX_SORT)
+        name = os.path.basename(args.mtx)
+    else:
+        A, vals = _rmat_all(sp, dt)
+        name = "R-MAT 2^20, 2^24 nnz"
+    x = sp.gen_vector(dt, A.ncols, seed=22)
+    rows = []
+    for rep in range(min(args.steps, 5)):
+        t0 = time.perf_counter()
+        plan = sp.Plan(ctx, "CSR-row", A)
+        ctx.sync()
+        t1 = time.perf_counter()
+        xd = sp.DeviceArray.from_numpy(ctx, x)
+        t2 = time.perf_counter()
+        yd = sp.DeviceArray(ctx, A.nrows, np.float32)
+        t2b = time.perf_counter()
+        plan.spmv(xd.addr, yd.addr)
+        ctx.sync()
+        t3 = time.perf_counter()
+        y = yd.numpy()
+        t4 = time.perf_counter()
+        rows.append((t1 - t0, t2 - t1, t3 - t2b, t4 - t3))
+        del plan
+    med = [statistics.median(r[i] for r in rows) * 1e3 for i in range(4)]
+    ok = parity_ok(orc, dt, A, x, y)
+    info_b = 146800644 if not args.mtx else None
+    log(f"[bench] phases {name}: Load Matrix {med[0]:.2f} ms, Load Input Vector {med[1]:.3f} ms, "
+        f"Kernel {med[2]:.3f} ms, Retrieve Output Vector {med[3]:.3f} ms, Merge 0 (on GPU), parity {ok}")
+    total = sum(med)
+    cfg = {"workload": f"five-phase end-to-end CSR-row SpMV, {name}, FP32, host buffers in and out "
+                       f"(PCIe-inclusive, wall clock)", "nnz": A.nnz}
+    return [line(args, total, total, "f32", cfg, unit="ms end to end", higher=False,
+                 extra={"load_matrix_ms": round(med[0], 3), "load_input_vector_ms": round(med[1], 4),
+                        "kernel_ms": round(med[2], 4), "retrieve_output_vector_ms": round(med[3], 4),
+                        "merge_ms": 0.0, "parity_vs_oracle": ok,
+                        "kernel_plus_transfers_gbs": round((info_b or 0) / ((med[1] + med[2] + med[3]) * 1e-3) / 1e9, 1)
+                        if info_b else None})]
 
 
 def wl_batched(sp, ctx, args, orc):
@@ -367,7 +413,7 @@ def wl_selector(sp, ctx, args, orc):
 
 
 WORKLOADS = {"rmat": wl_rmat, "stencil5": wl_stencil5, "formats": wl_formats, "stencil27": wl_stencil27,
-             "rmat2d": wl_rmat2d,
+             "rmat2d": wl_rmat2d, "phases": wl_phases,
              "batched": wl_batched, "selector": wl_selector}
 
 
@@ -385,9 +431,14 @@ def main():
     ap.add_argument("--nprob", type=int, default=256, help="batched: number of problems")
     ap.add_argument("--sel-count", type=int, default=40, help="selector: matrices evaluated")
     ap.add_argument("--sel-seed", type=int, default=55)
+    ap.add_argument("--dtypes", default=None,
+                    help="formats / stencil27: comma-separated dtypes (i8,i16,i32,i64,f32,f64); default all")
+    ap.add_argument("--mtx", default=None, help="phases: a Matrix Market file instead of the R-MAT matrix")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 
+    names = {v: k for k, v in DT_NAME.items()}
+    args.dtypes = [names[d] for d in args.dtypes.split(",")] if args.dtypes else list(range(6))
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build targets a single MI355X and has no "
               f"multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)

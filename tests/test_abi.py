@@ -2,6 +2,8 @@
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -37,3 +39,20 @@ def test_ctx_without_gpu_reports_error(sp):
     p = ctypes.c_void_p()
     rc = sp.lib().sp_ctx_create(0, ctypes.byref(p))
     assert rc in (-6, -4)
+
+
+def test_c_caller_of_the_host_surface(tmp_path):
+    """tests/c/host_surface_demo.c compiled with gcc against include/sparsep.h and libsparsep.so."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "demo")
+    libdir = os.path.join(root, "sparsep_amd")
+    subprocess.run(["gcc", "-O1", "-std=c11", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "c", "host_surface_demo.c"), "-o", exe, "-L", libdir, "-lsparsep",
+                    f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.startswith("ok")
