@@ -301,6 +301,42 @@ def wl_phases(sp, ctx, args, orc):
                         if info_b else None})]
 
 
+def wl_all25(sp, ctx, args, orc):
+    """Every kernel id with its default configuration on the 27-point stencil (64^3 x 4 DOF) in
+    the first of --dtypes that is FP32 or FP64 (default FP32)."""
+    import numpy as np
+    out = []
+    dt = next((d for d in args.dtypes if d in (sp.F32, sp.F64)), sp.F32)
+    A = sp.gen_stencil27(64, 4, True, dt, seed=3)
+    x = sp.gen_vector(dt, A.ncols, d=sp.dist(sp.DIST_NORMAL, 0, 1), seed=33)
+    xd = sp.DeviceArray.from_numpy(ctx, x)
+    yd = sp.DeviceArray(ctx, A.nrows, x.dtype)
+    csr_bytes = None
+    for kid, kname in enumerate(sp.KERNELS):
+        t0 = time.time()
+        plan = sp.Plan(ctx, kid, A)
+        info = plan.info()
+        build = time.time() - t0
+        mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
+        ok = parity_ok(orc, dt, A, x, yd.numpy())
+        if kid == 0:
+            csr_bytes = info["algo_bytes"]
+        gbs = info["algo_bytes"] / (mean_ms * 1e-3) / 1e9
+        log(f"[bench] all25 {DT_NAME[dt]} {kname:14s} {mean_ms * 1e3:8.2f} us {gbs:7.0f} GB/s "
+            f"(CSR-equivalent {csr_bytes / (mean_ms * 1e-3) / 1e9:6.0f}) parity {ok} build {build:.1f}s")
+        cfg = {"workload": f"64^3 x 4 DOF 27-point stencil, {kname}, default configuration", "kernel": kname,
+               "nnz": A.nnz, "algo_bytes": info["algo_bytes"], "vert": info["vert"]}
+        out.append(line(args, gbs, mean_ms, DT_NAME[dt], cfg,
+                        roofline={"bound": "hbm", "achieved": round(info["algo_bytes"] / (mean_dom * 1e-3) / 1e9, 2),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(info["algo_bytes"] / (mean_dom * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                  "traffic": None},
+                        extra={"parity_vs_oracle": ok, "launches": info["launches"], "build_s": round(build, 1),
+                               "csr_equivalent_gbs": round(csr_bytes / (mean_ms * 1e-3) / 1e9, 1)}))
+        del plan
+    return out
+
+
 def wl_batched(sp, ctx, args, orc):
     import numpy as np
     out = []
@@ -413,7 +449,7 @@ def wl_selector(sp, ctx, args, orc):
 
 
 WORKLOADS = {"rmat": wl_rmat, "stencil5": wl_stencil5, "formats": wl_formats, "stencil27": wl_stencil27,
-             "rmat2d": wl_rmat2d, "phases": wl_phases,
+             "rmat2d": wl_rmat2d, "phases": wl_phases, "all25": wl_all25,
              "batched": wl_batched, "selector": wl_selector}
 
 

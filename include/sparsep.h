@@ -531,6 +531,16 @@ int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int ite
 int sp_tune_best(const sp_tune_point *pts, int n, int kernel_id);
 int sp_tune_db_append(const char *path, const char *tag, const sp_tune_point *pts, int n);
 
+/* Conjugate gradient on a plan's matrix (square, symmetric positive
+ * definite; FP32 or FP64): solves A x = b for device vectors b, x (x holds
+ * the initial guess on entry).  Stops when ||b - A x|| / ||b|| <= rtol or
+ * after maxit iterations; *iters and *relres report where.  One plan (and
+ * its tuned kernel) serves every iteration; dot products reduce in a fixed
+ * order.  SP_ERR_STATE when p'Ap <= 0 (matrix not positive definite).  No
+ * counterpart in the reference: the iterative-solver caller its README
+ * names as the motivating use (README.md:7). */
+int sp_cg(sp_plan *A, const void *b, void *x, int maxit, double rtol, int *iters, double *relres);
+
 /* Bandwidth probe: time a device copy of `bytes` (read + write), returns
  * ms per copy; the measured stream-copy peak is 2*bytes/ms. */
 int sp_time_copy(sp_ctx *ctx, uint64_t bytes, int iters, float *ms);

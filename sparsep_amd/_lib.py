@@ -174,6 +174,7 @@ SIGNATURES = {
     "sp_batch_destroy": (I, [P]),
     "sp_batch_run_once": (I, [P, I, PP, PP, PP, ctypes.POINTER(ctypes.c_int)]),
     "sp_time_plans": (I, [P, I, PP, PP, PP, ctypes.POINTER(ctypes.c_int), I, U64, fp]),
+    "sp_cg": (I, [P, P, P, I, D, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
     "sp_features": (I, [ctypes.POINTER(sp_coo), fp]),
     "sp_mlp_create": (I, [P, I, I, I, I, U64, PP]),
     "sp_mlp_destroy": (I, [P]),

@@ -487,6 +487,13 @@ class Plan:
                                        flush_bytes, ms, md), "sp_time_spmv_split")
         return list(ms), list(md)
 
+    def cg(self, b_ptr, x_ptr, maxit=1000, rtol=1e-8):
+        """Conjugate gradient with this plan's matrix (sp_cg); x holds the start on entry."""
+        it, res = ctypes.c_int(), ctypes.c_double()
+        check(lib().sp_cg(self.ptr, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), maxit, rtol, ctypes.byref(it),
+                          ctypes.byref(res)), "sp_cg")
+        return it.value, res.value
+
     def time_spmm(self, k, X_ptr, ldx, Y_ptr, ldy, iters, flush_bytes=0):
         ms = (ctypes.c_float * iters)()
         check(lib().sp_time_spmm(self.ptr, k, ctypes.c_void_p(X_ptr), ldx, ctypes.c_void_p(Y_ptr), ldy,
