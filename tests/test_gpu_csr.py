@@ -78,7 +78,7 @@ def test_edge_cases(sp, ctx, dt):
 
 
 @pytest.mark.parametrize("dt", [2, 4, 5])
-@pytest.mark.parametrize("k", [2, 3, 4, 8, 11])
+@pytest.mark.parametrize("k", [2, 3, 4, 8, 11, 16, 32, 45])
 def test_spmm(sp, ctx, dt, k):
     A = sp.gen_rmat(11, 20000, dtype=dt, seed=12)
     X = np.stack([sp.gen_vector(dt, A.ncols, seed=20 + q) for q in range(k)], axis=1)

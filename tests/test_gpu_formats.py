@@ -57,8 +57,20 @@ def test_block_sizes(sp, ctx, rc, dt):
                 assert_parity(dt, y, A, x, f"{kern} {r}x{c} {kw}")
 
 
+@pytest.mark.parametrize("rc", [(1, 1), (2, 2), (2, 4), (4, 2), (4, 4), (8, 8)])
+@pytest.mark.parametrize("k", [16, 32, 37])
+def test_block_spmm_wide(sp, ctx, rc, k):
+    """Right-hand-side chunks of 16/32 in the register-blocked BCSR kernel (R*K <= 64)."""
+    r, c = rc
+    A = sp.gen_stencil27(6, 2, False, sp.F32, seed=5)
+    X = np.stack([sp.gen_vector(sp.F32, A.ncols, seed=60 + q) for q in range(k)], axis=1)
+    for kw in ({}, dict(lanes_per_row=1), dict(lanes_per_row=64)):
+        Y, _ = run(sp, ctx, "BCSR-nnz", A, X, sp.opts(row_block_size=r, col_block_size=c, **kw), k=k)
+        assert_parity_multi(sp.F32, Y, A, X, f"{r}x{c} {kw}")
+
+
 @pytest.mark.parametrize("dt", [2, 4])
-@pytest.mark.parametrize("k", [3, 8])
+@pytest.mark.parametrize("k", [3, 8, 19, 32])
 def test_spmm_all_kinds(sp, ctx, dt, k):
     A = sp.gen_rmat(11, 25000, dtype=dt, seed=21)
     X = np.stack([sp.gen_vector(dt, A.ncols, seed=40 + q) for q in range(k)], axis=1)
