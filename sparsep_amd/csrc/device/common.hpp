@@ -90,6 +90,59 @@ constexpr int WAVE = 64;
 
 // Uncached-at-L2-for-reuse hints are not used: every matrix stream is read
 // exactly once per SpMV and x is reused through L2/MALL.
-template <typename T> __device__ __forceinline__ T ldg(const T *p) { return __builtin_nontemporal_load(p); }
+template <typename P> __device__ __forceinline__ auto ldg(P p) { return __builtin_nontemporal_load(p); }
+
+// Global-address-space views of the pointers the kernels read from problem
+// descriptors: loads and stores through them compile to global_* rather than
+// flat_* instructions (flat memory operations also count against the LDS
+// wait counter, which serialises them with the kernels' LDS traffic).
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gptr<const T> gp(const T *p) { return (gptr<const T>)p; }
+template <typename T> __device__ __forceinline__ gptr<T> gp(T *p) { return (gptr<T>)p; }
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// N contiguous values from a stream read once, in the widest loads the size
+// allows (16- or 8-byte, non-temporal).  p must be aligned to the load width
+// (N * sizeof(T) a multiple of it and p at a multiple of N * sizeof(T)).
+template <typename T, int N> __device__ __forceinline__ void ldg_n(gptr<const T> p, T (&out)[N]) {
+    constexpr int B = N * (int)sizeof(T);
+    if constexpr (B % 16 == 0) {
+        u32x4 w[B / 16];
+#pragma unroll
+        for (int i = 0; i < B / 16; i++) w[i] = __builtin_nontemporal_load((gptr<const u32x4>)p + i);
+        __builtin_memcpy(out, w, B);
+    } else if constexpr (B % 8 == 0) {
+        u32x2 w[B / 8];
+#pragma unroll
+        for (int i = 0; i < B / 8; i++) w[i] = __builtin_nontemporal_load((gptr<const u32x2>)p + i);
+        __builtin_memcpy(out, w, B);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) out[i] = ldg(p + i);
+    }
+}
+
+// N contiguous values of a vector that is reused through the caches (x):
+// one wide load when the address is aligned to the vector's size, else
+// element loads.
+template <typename T, int N> __device__ __forceinline__ void ld_n(gptr<const T> p, T (&out)[N]) {
+    constexpr int B = N * (int)sizeof(T);
+    if constexpr (B == 16 || B == 8) {
+        if (((uintptr_t)p & (B - 1)) == 0) {
+            if constexpr (B == 16) {
+                const u32x4 w = *(gptr<const u32x4>)p;
+                __builtin_memcpy(out, &w, B);
+            } else {
+                const u32x2 w = *(gptr<const u32x2>)p;
+                __builtin_memcpy(out, &w, B);
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) out[i] = p[i];
+}
 
 }  // namespace spd

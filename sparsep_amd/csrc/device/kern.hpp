@@ -162,9 +162,10 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
     const uint32_t pid = unit_pid[t], ci = t + pid;
     const Prob P = probs[pid];
-    const T *__restrict__ x = io.x(pid, P);
-    T *__restrict__ y = io.y(pid, P);
-    const T *__restrict__ vals = (const T *)P.val;
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t i0 = crow[ci], j0 = cnnz[ci];
     const uint32_t na = crow[ci + 1] - i0, nb = cnnz[ci + 1] - j0;
@@ -177,13 +178,13 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
 #pragma unroll
     for (int q = 0; q <= IPT; q++) {
         const uint32_t k = tid + q * TPB;
-        if (k < nrl) re[q] = P.ptr[i0 + k + 1];
+        if (k < nrl) re[q] = rp[i0 + k + 1];
     }
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
         const uint32_t k = tid + q * TPB;
         if (k < nb) {
-            cv[q] = ldg(P.cind + j0 + k);
+            cv[q] = ldg(cip + j0 + k);
             vv[q] = ldg(vals + j0 + k);
         }
     }
@@ -276,15 +277,16 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
     for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const uint32_t pid = unit_pid[u];
     const Prob P = probs[pid];
-    const T *__restrict__ x = io.x(pid, P);
-    T *__restrict__ y = io.y(pid, P);
-    const T *__restrict__ vals = (const T *)P.val;
-    const uint32_t *__restrict__ colind = P.cind;
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    const gptr<const T> __restrict__ vals = gp((const T *)P.val);
+    const gptr<const uint32_t> __restrict__ colind = cip;
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
     const int lane = threadIdx.x % L, grp = threadIdx.x / L;
     for (uint32_t r = r0 + grp; r < r1; r += G) {
-        const uint32_t b = P.ptr[r], e = P.ptr[r + 1];
+        const uint32_t b = rp[r], e = rp[r + 1];
         A acc[K];
 #pragma unroll
         for (int q = 0; q < K; q++) acc[q] = A(0);
@@ -341,12 +343,13 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
     const uint32_t pid = unit_pid[t], ci = t + pid;
     const Prob P = probs[pid];
-    const T *__restrict__ x = io.x(pid, P);
-    T *__restrict__ y = io.y(pid, P);
-    const T *__restrict__ vals = (const T *)P.val;
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t e0 = cent[ci], n = cent[ci + 1] - e0;
-    const uint32_t next_row = (e0 + n < P.nent) ? P.rind[e0 + n] : NONE;
+    const uint32_t next_row = (e0 + n < P.nent) ? rip[e0 + n] : NONE;
 
     uint32_t rv[IPT], cv[IPT];
     T vv[IPT];
@@ -354,8 +357,8 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     for (int q = 0; q < IPT; q++) {
         const uint32_t k = tid + q * TPB;
         if (k < n) {
-            rv[q] = ldg(P.rind + e0 + k);
-            cv[q] = ldg(P.cind + e0 + k);
+            rv[q] = ldg(rip + e0 + k);
+            cv[q] = ldg(cip + e0 + k);
             vv[q] = ldg(vals + e0 + k);
         }
     }
@@ -462,25 +465,35 @@ __global__ __launch_bounds__(TPB) void bcsr_vector_kernel(const Prob *__restrict
     for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const uint32_t pid = unit_pid[u];
     const Prob P = probs[pid];
-    const T *__restrict__ x = io.x(pid, P);
-    T *__restrict__ y = io.y(pid, P);
-    const T *__restrict__ bv = (const T *)P.val;
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    const gptr<const T> __restrict__ bv = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
     const int lane = threadIdx.x % L, grp = threadIdx.x / L;
     const uint32_t ncols = P.ncols_elem;
     for (uint32_t I = r0 + grp; I < r1; I += G) {
-        const uint32_t b = P.ptr[I], e = P.ptr[I + 1];
+        const uint32_t b = rp[I], e = rp[I + 1];
         A acc[R][K];
 #pragma unroll
         for (int i = 0; i < R; i++)
 #pragma unroll
             for (int q = 0; q < K; q++) acc[i][q] = A(0);
         for (uint32_t k = b + lane; k < e; k += L) {
-            const uint32_t J = ldg(P.cind + k);
+            const uint32_t J = ldg(cip + k);
             T v[R * C];
+            ldg_n<T, R * C>(bv + (size_t)k * (R * C), v);
+            if (K == 1 && ldx == 1 && J * C + C <= ncols) {
+                // whole block column inside the matrix: the C x values in one load when aligned
+                T xs[C];
+                ld_n<T, C>(x + (size_t)J * C, xs);
 #pragma unroll
-            for (int z = 0; z < R * C; z++) v[z] = ldg(bv + (size_t)k * (R * C) + z);
+                for (int c = 0; c < C; c++)
+#pragma unroll
+                    for (int i = 0; i < R; i++) acc[i][0] = fma_acc<T>(v[i * C + c], xs[c], acc[i][0]);
+                continue;
+            }
 #pragma unroll
             for (int c = 0; c < C; c++) {
                 const uint32_t col = J * C + c;
@@ -538,19 +551,20 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
     for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
     const uint32_t pid = unit_pid[t], ci = t + pid;
     const Prob P = probs[pid];
-    const T *__restrict__ x = io.x(pid, P);
-    T *__restrict__ y = io.y(pid, P);
-    const T *__restrict__ bv = (const T *)P.val;
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    const gptr<const T> __restrict__ bv = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t b0 = cent[ci], n = cent[ci + 1] - b0;
-    const uint32_t next_brow = (b0 + n < P.nent) ? P.rind[b0 + n] : NONE;
+    const uint32_t next_brow = (b0 + n < P.nent) ? rip[b0 + n] : NONE;
 
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
         const uint32_t k = tid + q * TPB;
         if (k < n) {
-            s_brow[k] = ldg(P.rind + b0 + k);
-            s_bcol[k] = ldg(P.cind + b0 + k);
+            s_brow[k] = ldg(rip + b0 + k);
+            s_bcol[k] = ldg(cip + b0 + k);
         }
     }
     __syncthreads();
@@ -670,7 +684,7 @@ __global__ void fixup_kernel(const Prob *__restrict__ probs, const uint32_t *__r
         const Prob P = probs[pid];
         const uint32_t row = r * S + s;
         if (row >= P.nrows_elem) return;
-        T *yp = io.y(pid, P) + (size_t)row * io.ly(P);
+        const gptr<T> yp = gp(io.y(pid, P)) + (size_t)row * io.ly(P);
 #pragma unroll
         for (int q = 0; q < K; q++) yp[q] = to_val<T>((A)yp[q] + sum[q]);
         return;
@@ -679,7 +693,7 @@ __global__ void fixup_kernel(const Prob *__restrict__ probs, const uint32_t *__r
     if (z >= nzero) return;
     const uint32_t pid = zpid[z];
     const Prob P = probs[pid];
-    T *yp = io.y(pid, P) + (size_t)zrow[z] * io.ly(P);
+    const gptr<T> yp = gp(io.y(pid, P)) + (size_t)zrow[z] * io.ly(P);
 #pragma unroll
     for (int q = 0; q < K; q++) yp[q] = T(0);
 }
