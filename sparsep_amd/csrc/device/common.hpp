@@ -129,15 +129,18 @@ template <typename T, int N> __device__ __forceinline__ void ldg_n(gptr<const T>
 // element loads.
 template <typename T, int N> __device__ __forceinline__ void ld_n(gptr<const T> p, T (&out)[N]) {
     constexpr int B = N * (int)sizeof(T);
-    if constexpr (B == 16 || B == 8) {
-        if (((uintptr_t)p & (B - 1)) == 0) {
-            if constexpr (B == 16) {
-                const u32x4 w = *(gptr<const u32x4>)p;
-                __builtin_memcpy(out, &w, B);
-            } else {
-                const u32x2 w = *(gptr<const u32x2>)p;
-                __builtin_memcpy(out, &w, B);
-            }
+    if constexpr (B % 16 == 0) {
+        if (((uintptr_t)p & 15) == 0) {
+            u32x4 w[B / 16];
+#pragma unroll
+            for (int i = 0; i < B / 16; i++) w[i] = ((gptr<const u32x4>)p)[i];
+            __builtin_memcpy(out, w, B);
+            return;
+        }
+    } else if constexpr (B == 8) {
+        if (((uintptr_t)p & 7) == 0) {
+            const u32x2 w = *(gptr<const u32x2>)p;
+            __builtin_memcpy(out, &w, B);
             return;
         }
     }

@@ -96,9 +96,13 @@ void merge_coords(const uint32_t *rp, uint32_t nrows, uint32_t nnz, uint32_t til
 
 // Row units of one row range [r0, r1) of a CSR (or BCSR block-row) pointer
 // array: the tasklet-level split of the reference (by rows or by entries,
-// with its row granularity), one unit per tasklet.
+// with its row granularity), one unit per tasklet.  A unit longer than
+// max_rows rows is cut into even pieces: the nnz-balanced split puts every
+// trailing empty row into its last tasklet (a 2D stripe of a banded matrix
+// can leave half the rows empty), and one workgroup walking those rows
+// would serialise the launch.
 int row_units(const uint32_t *rp, uint32_t r0, uint32_t r1, uint32_t ntasklets, int by_nnz, uint32_t gran,
-              std::vector<uint32_t> &out) {
+              uint32_t max_rows, std::vector<uint32_t> &out) {
     const uint32_t rows = r1 - r0;
     if (rows == 0) return SP_OK;
     std::vector<uint32_t> split(ntasklets + 1);
@@ -106,7 +110,14 @@ int row_units(const uint32_t *rp, uint32_t r0, uint32_t r1, uint32_t ntasklets, 
                     : sp_partition_tsklt_by_row(rows, ntasklets, gran, split.data());
     if (rc) return rc;
     split[ntasklets] = rows;  // the reference can leave trailing rows uncovered; cover them
-    for (uint32_t i = 0; i <= ntasklets; i++) push_cut(out, r0 + std::min(split[i], rows));
+    uint32_t prev = 0;
+    for (uint32_t i = 0; i <= ntasklets; i++) {
+        const uint32_t cut = std::max(prev, std::min(split[i], rows));
+        if (max_rows && cut - prev > max_rows)
+            for (uint32_t s = prev + max_rows; s < cut; s += max_rows) push_cut(out, r0 + s);
+        push_cut(out, r0 + cut);
+        prev = cut;
+    }
     return SP_OK;
 }
 
@@ -197,7 +208,9 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
             const uint32_t r0 = split[q], r1 = std::max(split[q + 1], r0);
             const uint32_t T = p->opts.nsub ? p->opts.nsub
                                             : (uint32_t)std::max<uint64_t>(1, (r1 - r0 + rows_per_unit - 1) / rows_per_unit);
-            if ((rc = row_units(A->rowptr, r0, r1, T, p->opts.sub_balance == SP_BAL_NNZ, gran, units))) return rc;
+            if ((rc = row_units(A->rowptr, r0, r1, T, p->opts.sub_balance == SP_BAL_NNZ, gran,
+                                (uint32_t)(4 * rows_per_unit), units)))
+                return rc;
         }
         push_cut(units, A->nrows);
         if (A->nrows == 0) units.clear();
@@ -647,7 +660,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
                             if (r1 == r0) continue;
                             const uint32_t Tn = p->opts.nsub ? p->opts.nsub
                                                              : (uint32_t)std::max<uint64_t>(1, (r1 - r0 + rpu - 1) / rpu);
-                            rc = row_units(hrp, r0, r1, Tn, p->opts.sub_balance == SP_BAL_NNZ, gran, k0);
+                            rc = row_units(hrp, r0, r1, Tn, p->opts.sub_balance == SP_BAL_NNZ, gran, (uint32_t)(4 * rpu), k0);
                         }
                         push_cut(k0, rows);
                     }
@@ -715,6 +728,20 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             b.add(P, k0, kind == WK_CSR_MERGE ? &k1 : nullptr);
         }
         if (rc) break;
+        // algorithmic bytes: the tile formats' arrays, x once and y once (the partial
+        // vectors and their reduction are overhead, not algorithm)
+        {
+            const uint64_t xy = ((uint64_t)n + A0->ncols) * sv;
+            if (tk.format == SP_FMT_CSR)
+                p->algo_bytes = (uint64_t)nt * (th + 1) * 4 + (uint64_t)T->nnz * (4 + sv) + xy;
+            else if (tk.format == SP_FMT_COO)
+                p->algo_bytes = (uint64_t)T->nnz * (8 + sv) + xy;
+            else if (tk.format == SP_FMT_BCSR)
+                p->algo_bytes = T->num_blocks * ((uint64_t)R * C * sv + 4) +
+                                (uint64_t)nt * (T->num_block_rows + 1) * 4 + xy;
+            else
+                p->algo_bytes = T->num_blocks * ((uint64_t)R * C * sv + 8) + xy;
+        }
         const bool carries = kind == WK_CSR_MERGE || kind == WK_COO_SEG || kind == WK_BCOO_SEG;
         rc = b.finish(kind, carries, kind == WK_BCOO_SEG ? R : 1);
         p->work.zero_all = (kind == WK_COO_SEG || kind == WK_BCOO_SEG);
