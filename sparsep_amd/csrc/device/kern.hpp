@@ -164,7 +164,7 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     const Prob P = probs[pid];
     const gptr<const T> __restrict__ x = gp(io.x(pid, P));
     const gptr<T> __restrict__ y = gp(io.y(pid, P));
-    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    [[maybe_unused]] const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
     const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t i0 = crow[ci], j0 = cnnz[ci];
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
     const Prob P = probs[pid];
     const gptr<const T> __restrict__ x = gp(io.x(pid, P));
     const gptr<T> __restrict__ y = gp(io.y(pid, P));
-    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    [[maybe_unused]] const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
     const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const gptr<const uint32_t> __restrict__ colind = cip;
     const int ldx = io.lx(P), ldy = io.ly(P);
@@ -291,6 +291,20 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
 #pragma unroll
         for (int q = 0; q < K; q++) acc[q] = A(0);
         uint32_t k = b + lane;
+        // four entries per lane in flight (index and value loads, then the four gathers)
+        // before any is consumed: long rows are otherwise bound by load latency
+        if constexpr (K == 1) {
+            for (; k + 3 * L < e; k += 4 * L) {
+                uint32_t c[4];
+                T v[4], xv[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) { c[j] = ldg(colind + k + j * L); v[j] = ldg(vals + k + j * L); }
+#pragma unroll
+                for (int j = 0; j < 4; j++) xv[j] = x[(size_t)c[j] * ldx];
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[0] = fma_acc<T>(v[j], xv[j], acc[0]);
+            }
+        }
         for (; k + L < e; k += 2 * L) {
             const uint32_t c0 = ldg(colind + k), c1 = ldg(colind + k + L);
             const T v0 = ldg(vals + k), v1 = ldg(vals + k + L);
@@ -345,7 +359,7 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     const Prob P = probs[pid];
     const gptr<const T> __restrict__ x = gp(io.x(pid, P));
     const gptr<T> __restrict__ y = gp(io.y(pid, P));
-    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    [[maybe_unused]] const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
     const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t e0 = cent[ci], n = cent[ci + 1] - e0;
@@ -467,7 +481,7 @@ __global__ __launch_bounds__(TPB) void bcsr_vector_kernel(const Prob *__restrict
     const Prob P = probs[pid];
     const gptr<const T> __restrict__ x = gp(io.x(pid, P));
     const gptr<T> __restrict__ y = gp(io.y(pid, P));
-    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    [[maybe_unused]] const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
     const gptr<const T> __restrict__ bv = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
@@ -553,7 +567,7 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
     const Prob P = probs[pid];
     const gptr<const T> __restrict__ x = gp(io.x(pid, P));
     const gptr<T> __restrict__ y = gp(io.y(pid, P));
-    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
+    [[maybe_unused]] const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
     const gptr<const T> __restrict__ bv = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t b0 = cent[ci], n = cent[ci + 1] - b0;
@@ -568,20 +582,33 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
         }
     }
     __syncthreads();
-    // phase 1: element (blk, i, c) handled by thread tid + q*TPB
-    const uint32_t nel = n * (R * C);
-    for (uint32_t base = 0; base < (uint32_t)TILE * R * C; base += TPB) {
-        const uint32_t e = base + tid;
-        const uint32_t blk = e / (R * C), rem = e % (R * C), i = rem / C, c = rem % C;
-        A p = A(0);
-        if (e < nel) {
-            const uint32_t col = s_bcol[blk] * C + c;
-            if (col < P.ncols_elem) p = mul<T>(ldg(bv + (size_t)b0 * (R * C) + e), x[(size_t)col * ldx]);
+    // phase 1: a thread takes whole blocks (R*C contiguous values in wide loads,
+    // the C x values in one load when aligned) and forms their R row sums
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const uint32_t blk = tid + q * TPB;
+        if (blk < n) {
+            const uint32_t J = s_bcol[blk];
+            T v[R * C];
+            ldg_n<T, R * C>(bv + (size_t)(b0 + blk) * (R * C), v);
+            T xs[C];
+            if (ldx == 1 && J * C + C <= P.ncols_elem) {
+                ld_n<T, C>(x + (size_t)J * C, xs);
+            } else {
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const uint32_t col = J * C + c;
+                    xs[c] = col < P.ncols_elem ? x[(size_t)col * ldx] : T(0);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                A p = A(0);
+#pragma unroll
+                for (int c = 0; c < C; c++) p = p + mul<T>(v[i * C + c], xs[c]);
+                s_part[blk * R + i] = p;
+            }
         }
-        // sum over the C lanes of this block row (fixed order: butterfly)
-        p = group_sum<C>(p);
-        if (c == 0 && e < nel) s_part[blk * R + i] = p;
-        if (base + TPB >= nel) break;
     }
     __syncthreads();
 
@@ -670,18 +697,24 @@ __global__ void fixup_kernel(const Prob *__restrict__ probs, const uint32_t *__r
     if (g < ncarry) {
         const uint32_t s = (uint32_t)(g / nunits), t = (uint32_t)(g % nunits);
         const uint32_t *cr = carry_row + (size_t)s * nunits;
+        const A *cv = carry_val + (size_t)s * nunits * K;
+        // Loads that depend only on t go out together (the kernel is a chain of
+        // dependent loads, so its time is their latency): this unit's carry,
+        // its neighbour's, and the carried value (every unit writes one).
         const uint32_t r = cr[t];
-        if (r == NONE) return;
         const uint32_t pid = unit_pid[t];
-        if (t > 0 && unit_pid[t - 1] == pid && cr[t - 1] == r) return;
+        const uint32_t rprev = t > 0 ? cr[t - 1] : NONE;
+        const uint32_t pprev = t > 0 ? unit_pid[t - 1] : NONE;
         A sum[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) sum[q] = A(0);
-        for (uint32_t u = t; u < nunits && unit_pid[u] == pid && cr[u] == r; u++) {
-#pragma unroll
-            for (int q = 0; q < K; q++) sum[q] = sum[q] + carry_val[((size_t)s * nunits + u) * K + q];
-        }
+        for (int q = 0; q < K; q++) sum[q] = A(0) + cv[(size_t)t * K + q];
         const Prob P = probs[pid];
+        if (r == NONE) return;
+        if (pprev == pid && rprev == r) return;  // not the first unit of this run
+        for (uint32_t u = t + 1; u < nunits && unit_pid[u] == pid && cr[u] == r; u++) {
+#pragma unroll
+            for (int q = 0; q < K; q++) sum[q] = sum[q] + cv[(size_t)u * K + q];
+        }
         const uint32_t row = r * S + s;
         if (row >= P.nrows_elem) return;
         const gptr<T> yp = gp(io.y(pid, P)) + (size_t)row * io.ly(P);
