@@ -81,15 +81,22 @@ def wl_rmat(sp, ctx, args, orc):
     A = sp.gen_rmat(args.scale, 1 << args.nnz_log2, 0.57, 0.19, 0.19, dtype=dt, seed=2)
     x = sp.gen_vector(dt, A.ncols, seed=22)
     log(f"[bench] R-MAT 2^{args.scale}: nnz={A.nnz} generated in {time.time() - t0:.1f}s")
-    # the library's own autotuner picks the CSR-nnz configuration (outside the timed region)
-    pts = sp.tune(ctx, A, sp.tune_space(A, sp.KID["CSR-nnz"]), iters=3, flush_bytes=args.flush_mb << 20)
-    best = sp.tune_best(pts)
-    o = sp.point_opts(pts[best]) if best >= 0 else sp.opts(algo=sp.ALGO_SEGMENTED, items_per_lane=8)
+    if args.algo < 0:
+        # the library's own autotuner picks the CSR-nnz configuration (outside the timed region)
+        pts = sp.tune(ctx, A, sp.tune_space(A, sp.KID["CSR-nnz"]), iters=3, flush_bytes=args.flush_mb << 20)
+        best = sp.tune_best(pts)
+        o = sp.point_opts(pts[best]) if best >= 0 else sp.opts(algo=sp.ALGO_SEGMENTED, items_per_lane=8)
+        chosen = "chosen by the autotuner"
+    else:
+        o = sp.opts(algo=args.algo, items_per_lane=args.ipt)
+        chosen = "fixed by --algo/--ipt"
     plan = sp.Plan(ctx, "CSR-nnz", A, o)
     pinfo = plan.info()
-    algo_name = (f"merge-path, {pinfo['items_per_lane']} items per lane" if pinfo["algo"] == sp.ALGO_SEGMENTED
-                 else f"vector, {pinfo['lanes_per_row']} lanes per row")
-    log(f"[bench] tuned CSR-nnz configuration: {algo_name}")
+    algo_name = {sp.ALGO_SEGMENTED: f"merge-path, {pinfo['items_per_lane']} items per lane",
+                 sp.ALGO_SEGMENTED_SORTED: f"merge-path with column-ordered tiles, {pinfo['items_per_lane']} "
+                                           "items per lane"}.get(pinfo["algo"],
+                                                                 f"vector, {pinfo['lanes_per_row']} lanes per row")
+    log(f"[bench] CSR-nnz configuration ({chosen}): {algo_name}")
     algo_bytes = pinfo["algo_bytes"]
     xd = sp.DeviceArray.from_numpy(ctx, x)
     yd = sp.DeviceArray(ctx, A.nrows, np.float32)
@@ -120,7 +127,7 @@ def wl_rmat(sp, ctx, args, orc):
                "host_cpus": orc.num_cpus()}
         log(f"[bench] cpu baseline {cms_cpu:.1f} ms/run ({cpu_gbs:.2f} GB/s, {iters} runs)")
     cfg = {"workload": f"R-MAT 2^{args.scale} x 2^{args.scale}, {A.nnz} unique nnz, a=0.57 b=0.19 c=0.19, "
-                       f"FP32 CSR SpMV (CSR-nnz, {algo_name}, chosen by the autotuner), "
+                       f"FP32 CSR SpMV (CSR-nnz, {algo_name}, {chosen}), "
                        f"MALL flushed before each step",
            "nrows": A.nrows, "nnz": A.nnz, "algo_bytes": algo_bytes,
            "gflops": round(2 * A.nnz / (mean_ms * 1e-3) / 1e9, 2)}
@@ -128,9 +135,10 @@ def wl_rmat(sp, ctx, args, orc):
     # (FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy kernel of the same runs)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "csr_merge_rmat20_f32_traffic.json")
-    if os.path.exists(tpath) and pinfo["algo"] == sp.ALGO_SEGMENTED and args.scale == 20 and args.nnz_log2 == 24:
+    tkey = {sp.ALGO_SEGMENTED: "merge", sp.ALGO_SEGMENTED_SORTED: "sorted"}.get(pinfo["algo"])
+    if os.path.exists(tpath) and tkey and args.scale == 20 and args.nnz_log2 == 24:
         with open(tpath) as f:
-            t = json.load(f)["per_items_per_lane"].get(str(pinfo["items_per_lane"]))
+            t = json.load(f)["per_config"].get(f"{tkey}{pinfo['items_per_lane']}")
         traffic = t["traffic_bytes"] if t else None
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
@@ -211,24 +219,30 @@ def wl_formats(sp, ctx, args, orc):
 
 def wl_stencil27(sp, ctx, args, orc):
     out = []
-    kinds = ["CSR-nnz", "BCSR-block", "DCSR", "DCOO", "RBDCSR", "RBDBCSR-block", "BDCSR", "BDBCOO-block"]
+    # (kernel, block edge); the 4 x 4 blocks match the stencil's 4 unknowns per grid point
+    kinds = [("CSR-nnz", 2), ("BCSR-block", 2), ("BCSR-block", 4), ("BCOO-block", 4), ("DCSR", 2), ("DCOO", 2),
+             ("RBDCSR", 2), ("RBDBCSR-block", 2), ("BDCSR", 2), ("BDBCOO-block", 2), ("BDBCOO-block", 4)]
     for dt in [d for d in args.dtypes if d in (sp.F32, sp.F64)]:
         A = sp.gen_stencil27(64, 4, True, dt, seed=3)
         x = sp.gen_vector(dt, A.ncols, d=sp.dist(sp.DIST_NORMAL, 0, 1), seed=33)
         xd = sp.DeviceArray.from_numpy(ctx, x)
         yd = sp.DeviceArray(ctx, A.nrows, x.dtype)
-        for kname in kinds:
+        for kname, be in kinds:
             t0 = time.time()
-            plan = sp.Plan(ctx, kname, A, sp.opts(vert=args.vert) if kname[0] in "DRB" and kname[1] != "C" else None)
+            kw = dict(row_block_size=be, col_block_size=be)
+            if kname[0] in "DRB" and kname[1] != "C":
+                kw["vert"] = args.vert
+            plan = sp.Plan(ctx, kname, A, sp.opts(**kw))
             info = plan.info()
             build = time.time() - t0
             mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
             ok = parity_ok(orc, dt, A, x, yd.numpy())
             gbs = info["algo_bytes"] / (mean_dom * 1e-3) / 1e9
-            log(f"[bench] stencil27 {DT_NAME[dt]} {kname:13s} {mean_ms * 1e3:8.2f} us {gbs:7.0f} GB/s "
+            tag = f"{kname} {be}x{be}" if "B" in kname.replace("BD", "") else kname
+            log(f"[bench] stencil27 {DT_NAME[dt]} {tag:17s} {mean_ms * 1e3:8.2f} us {gbs:7.0f} GB/s "
                 f"(dominant) parity {ok} build {build:.1f}s")
-            cfg = {"workload": f"64^3 x 4 DOF 27-point stencil, periodic, {kname}", "kernel": kname, "nnz": A.nnz,
-                   "algo_bytes": info["algo_bytes"], "vert": info.get("vert", 0)}
+            cfg = {"workload": f"64^3 x 4 DOF 27-point stencil, periodic, {tag}", "kernel": kname, "nnz": A.nnz,
+                   "block": [be, be], "algo_bytes": info["algo_bytes"], "vert": info.get("vert", 0)}
             out.append(line(args, info["algo_bytes"] / (mean_ms * 1e-3) / 1e9, mean_ms, DT_NAME[dt], cfg,
                             roofline={"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                                       "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
@@ -491,6 +505,10 @@ def main():
                     help="formats / stencil27: comma-separated dtypes (i8,i16,i32,i64,f32,f64); default all")
     ap.add_argument("--mtx", default=None, help="phases: a Matrix Market file instead of the R-MAT matrix")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--algo", type=int, default=-1,
+                    help="rmat: fixed CSR algorithm (2 merge path, 3 column-ordered merge tiles) instead of the "
+                         "autotuner's pick, for profiler runs")
+    ap.add_argument("--ipt", type=int, default=16, help="rmat: items per lane with --algo")
     args = ap.parse_args()
 
     names = {v: k for k, v in DT_NAME.items()}

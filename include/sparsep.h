@@ -395,7 +395,12 @@ const char *sp_kernel_name(int id);
 typedef enum sp_algo {
     SP_ALGO_AUTO = 0,
     SP_ALGO_VECTOR = 1,    /* row granular: `lanes_per_row` lanes per row   */
-    SP_ALGO_SEGMENTED = 2  /* entry balanced: fixed entry tiles, rows split */
+    SP_ALGO_SEGMENTED = 2, /* entry balanced: fixed entry tiles, rows split */
+    /* SEGMENTED with every tile's entries stored in column order, so the
+     * lanes of one gather instruction share cache lines; products return to
+     * row order in LDS, giving the same sums in the same order.  CSR ids
+     * 0 and 1 only; needs ncols <= 2^32 / (256 * items_per_lane). */
+    SP_ALGO_SEGMENTED_SORTED = 3
 } sp_algo;
 
 /* Tasklet-level balance (the reference BLNC_TSKLT_* knobs). */

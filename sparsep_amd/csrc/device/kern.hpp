@@ -42,6 +42,8 @@ struct Prob {
     uint32_t ncols_elem;   // element columns
     int64_t xoff, yoff;    // element offsets into the per-call x / y bases
     int32_t ldx, ldy;      // leading dimensions when x / y come from pointer tables
+    uint32_t pbits;        // CSR merge tiles in column order: cind holds col << pbits | position in tile
+    uint32_t pad;
 };
 
 template <typename T> struct Ptrs {
@@ -169,6 +171,8 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t i0 = crow[ci], j0 = cnnz[ci];
     const uint32_t na = crow[ci + 1] - i0, nb = cnnz[ci + 1] - j0;
+    // tiles stored in column order carry each entry's row-order position in the low bits
+    const uint32_t pb = P.pbits, pmask = (1u << pb) - 1;
 
     // All loads of the tile are issued before the first is consumed, then
     // all x gathers: two memory round trips per tile.
@@ -192,16 +196,17 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
         T xv[IPT];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < nb) xv[q] = x[(size_t)cv[q] * ldx];
+            if (tid + q * TPB < nb) xv[q] = x[(size_t)(cv[q] >> pb) * ldx];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < nb) s_prod[tid + q * TPB] = mul<T>(vv[q], xv[q]);
+            if (tid + q * TPB < nb) s_prod[pb ? (cv[q] & pmask) : tid + q * TPB] = mul<T>(vv[q], xv[q]);
     } else {
 #pragma unroll
         for (int q = 0; q < IPT; q++)
             if (tid + q * TPB < nb) {
-                s_col[tid + q * TPB] = cv[q];
-                s_val[tid + q * TPB] = vv[q];
+                const uint32_t pos = pb ? (cv[q] & pmask) : tid + q * TPB;
+                s_col[pos] = cv[q] >> pb;
+                s_val[pos] = vv[q];
             }
     }
 #pragma unroll
@@ -540,6 +545,96 @@ __global__ __launch_bounds__(TPB) void bcsr_vector_kernel(const Prob *__restrict
 }
 
 // ---------------------------------------------------------------------------
+// BCSR, coalesced (one right-hand side): a wave takes one block row at a time
+// and its lanes split every block into 16-byte chunks, so one load
+// instruction reads 64 consecutive chunks (1 KB of consecutive blocks) where
+// the register-blocked kernel touches 64 separate blocks.  A chunk lies in
+// one block row (C * sizeof(T) is a multiple of 16); its lane multiplies it
+// with the matching 16 bytes of x, and a butterfly over the lanes that share
+// a row forms the row sums.
+// ---------------------------------------------------------------------------
+template <typename T, int R, int C> struct Coal {
+    static constexpr int CH = 16 / (int)sizeof(T);  // elements per chunk
+    static constexpr int CPR = C / CH;              // chunks per block row
+    static constexpr int G = R * CPR;               // chunks per block
+    static constexpr int PB = WAVE / G;             // blocks per wave step
+    static constexpr bool ok = (C % CH == 0) && G <= WAVE && (WAVE % G == 0);
+};
+
+// v += lane_xor<M>(v) for every power-of-two M in [lo, hi)
+template <int M, int HI, typename A> __device__ __forceinline__ A xor_sum(A v) {
+    if constexpr (M < HI) {
+        v += lane_xor<M>(v);
+        return xor_sum<M * 2, HI>(v);
+    } else {
+        return v;
+    }
+}
+
+template <typename T, int R, int C>
+__global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__ probs,
+                                                        const uint32_t *__restrict__ unit_pid,
+                                                        const uint32_t *__restrict__ urow, Ptrs<T> io,
+                                                        uint32_t nunits) {
+    using A = acc_t<T>;
+    using CO = Coal<T, R, C>;
+    static_assert(CO::ok, "block shape not eligible");
+    constexpr int NW = TPB / WAVE;
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    const int blk_off = lane / CO::G, g = lane % CO::G, i = g / CO::CPR, c0 = (g % CO::CPR) * CO::CH;
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint32_t pid = unit_pid[u];
+    const Prob P = probs[pid];
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), cip = gp(P.cind);
+    const gptr<const T> __restrict__ bv = gp((const T *)P.val);
+    const int ldx = io.lx(P), ldy = io.ly(P);
+    const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    const uint32_t ncols = P.ncols_elem;
+    for (uint32_t I = r0 + wid; I < r1; I += NW) {
+        const uint32_t b = rp[I], e = rp[I + 1];
+        A acc = A(0);
+        // U wave steps per iteration with all loads issued first; lanes past
+        // the row's end reload its last block and drop the product
+        constexpr int U = 4;
+        for (uint32_t k = b + blk_off; k < e; k += U * CO::PB) {
+            uint32_t J[U];
+            T v[U][CO::CH], xs[U][CO::CH];
+#pragma unroll
+            for (int u = 0; u < U; u++) J[u] = ldg(cip + min(k + u * CO::PB, e - 1));
+#pragma unroll
+            for (int u = 0; u < U; u++) ldg_n<T, CO::CH>(bv + (size_t)min(k + u * CO::PB, e - 1) * (R * C) + g * CO::CH, v[u]);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t col = J[u] * C + c0;
+                if (ldx == 1 && col + CO::CH <= ncols) {
+                    ld_n<T, CO::CH>(x + col, xs[u]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < CO::CH; j++)
+                        xs[u][j] = col + j < ncols ? x[(size_t)(col + j) * ldx] : T(0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (k + u * CO::PB < e) {
+#pragma unroll
+                    for (int j = 0; j < CO::CH; j++) acc = fma_acc<T>(v[u][j], xs[u][j], acc);
+                }
+        }
+        // lanes of row i: all chunks of the row (low bits) and all block slots (high bits)
+        acc = xor_sum<1, CO::CPR>(acc);
+        acc = xor_sum<CO::G, WAVE>(acc);
+        if (blk_off == 0 && (g % CO::CPR) == 0) {
+            const uint32_t row = I * R + i;
+            if (row < P.nrows_elem) y[(size_t)row * ldy] = to_val<T>(acc);
+        }
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // BCOO, block balanced: phase 1 forms the R row sums of every block of the
 // tile with lane groups of C over the block's elements (coalesced value
 // loads); phase 2 is the COO segmented walk over blocks with R-wide sums.
@@ -582,6 +677,37 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
         }
     }
     __syncthreads();
+    using CO = Coal<T, R, C>;
+    if constexpr (CO::ok && CO::G > 1) {
+        // phase 1, blocks wider than 16 bytes: lanes take 16-byte chunks of
+        // consecutive blocks (one load instruction covers 1 KB of values), and
+        // the lanes holding one block row's chunks sum them
+        if (n) {
+            const uint32_t nl = n - 1;
+            const int g = tid % CO::G;  // TPB is a multiple of G: a lane keeps its chunk slot
+            const int c0 = (g % CO::CPR) * CO::CH;
+#pragma unroll
+            for (int q = 0; q < IPT * CO::G; q++) {
+                const uint32_t blk = (uint32_t)(tid + q * TPB) / CO::G;
+                const uint32_t bc = min(blk, nl);  // lanes past the tile reload its last block
+                const uint32_t col = s_bcol[bc] * C + c0;
+                T v[CO::CH], xs[CO::CH];
+                ldg_n<T, CO::CH>(bv + (size_t)(b0 + bc) * (R * C) + g * CO::CH, v);
+                if (ldx == 1 && col + CO::CH <= P.ncols_elem) {
+                    ld_n<T, CO::CH>(x + col, xs);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < CO::CH; j++)
+                        xs[j] = col + j < P.ncols_elem ? x[(size_t)(col + j) * ldx] : T(0);
+                }
+                A p = A(0);
+#pragma unroll
+                for (int j = 0; j < CO::CH; j++) p = p + mul<T>(v[j], xs[j]);
+                p = xor_sum<1, CO::CPR>(p);
+                if (blk < n && g % CO::CPR == 0) s_part[blk * R + g / CO::CPR] = p;
+            }
+        }
+    } else {
     // phase 1: a thread takes whole blocks (R*C contiguous values in wide loads,
     // the C x values in one load when aligned) and forms their R row sums
 #pragma unroll
@@ -609,6 +735,7 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
                 s_part[blk * R + i] = p;
             }
         }
+    }
     }
     __syncthreads();
 

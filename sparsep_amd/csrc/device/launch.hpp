@@ -27,8 +27,9 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
             return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
         }
         if (w.kind == WK_BCSR_VEC) {
+            const uint32_t lanes = w.lanes ? w.lanes : 16;  // coalesced plans: 16 lanes for several RHS
 #define SPD_BLW(Rv, Cv, Lv) \
-    if (w.R == Rv && w.C == Cv && w.lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP; }
+    if (w.R == Rv && w.C == Cv && lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP; }
 #define SPD_BRCW(Rv, Cv) SPD_BLW(Rv, Cv, 1) SPD_BLW(Rv, Cv, 4) SPD_BLW(Rv, Cv, 16) SPD_BLW(Rv, Cv, 64)
             if constexpr (K <= 32) { SPD_BRCW(1, 1) SPD_BRCW(2, 2) SPD_BRCW(2, 4) }
             if constexpr (K <= 16) { SPD_BRCW(4, 4) SPD_BRCW(4, 2) }
@@ -40,7 +41,6 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
     } else {
     switch (w.kind) {
     case WK_CSR_MERGE:
-        if (K > 1 && w.ipt > 8) return SP_ERR_UNSUPPORTED;
         switch (w.ipt) {
         case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
         case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
@@ -57,7 +57,6 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
 #undef SPD_VEC
         break;
     case WK_COO_SEG:
-        if (K > 1 && w.ipt > 8) return SP_ERR_UNSUPPORTED;
         switch (w.ipt) {
         case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
         case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
@@ -66,8 +65,21 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         }
         break;
     case WK_BCSR_VEC: {
+        if (w.lanes == 0 && K == 1) {
+            // coalesced kernel (the plan checked the shape against the value size)
+#define SPD_CO(Rv, Cv) \
+    if constexpr (Coal<T, Rv, Cv>::ok) \
+        if (w.R == Rv && w.C == Cv) { hipLaunchKernelGGL((bcsr_coal_kernel<T, Rv, Cv>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break; }
+            do {
+                SPD_CO(2, 2) SPD_CO(4, 4) SPD_CO(8, 8) SPD_CO(2, 4) SPD_CO(4, 2)
+                return SP_ERR_UNSUPPORTED;
+            } while (0);
+#undef SPD_CO
+            break;
+        }
+        const uint32_t lanes = w.lanes ? w.lanes : 16;
 #define SPD_BL(Rv, Cv, Lv) \
-    if (w.R == Rv && w.C == Cv && w.lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break; }
+    if (w.R == Rv && w.C == Cv && lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break; }
 #define SPD_BRC(Rv, Cv) SPD_BL(Rv, Cv, 1) SPD_BL(Rv, Cv, 4) SPD_BL(Rv, Cv, 16) SPD_BL(Rv, Cv, 64)
         do {
             SPD_BRC(1, 1) SPD_BRC(2, 2) SPD_BRC(4, 4) SPD_BRC(8, 8) SPD_BRC(2, 4) SPD_BRC(4, 2)

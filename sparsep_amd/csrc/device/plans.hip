@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "plan.hpp"
@@ -137,6 +138,19 @@ uint32_t auto_lanes_1d(double avg) {
     return L;
 }
 
+// Shapes the coalesced BCSR kernel takes: a block row holds whole 16-byte
+// chunks and a block at most 64 of them (see spd::Coal).  It gives a whole
+// wave to each block row, so it is chosen only when the average block row
+// fills at least half a wave with chunks.
+bool coal_ok(int dtype, uint32_t R, uint32_t C, double avg_blocks) {
+    const uint32_t ch = 16 / (uint32_t)sp_dtype_size(dtype);
+    if (C % ch) return false;
+    const uint32_t g = R * (C / ch);
+    const bool shape = (R == 2 && C == 2) || (R == 4 && C == 4) || (R == 8 && C == 8) || (R == 2 && C == 4) ||
+                       (R == 4 && C == 2);
+    return shape && g <= 64 && 64 % g == 0 && avg_blocks * g >= 32.0;
+}
+
 uint32_t pick_ipt(const sp_opts &o, uint32_t def) {
     const uint32_t I = o.items_per_lane ? o.items_per_lane : def;
     return (I == 4 || I == 8 || I == 16) ? I : 0;
@@ -147,6 +161,33 @@ void tile_range(uint32_t e0, uint32_t e1, uint32_t tile, std::vector<uint32_t> &
     push_cut(out, e0);
     for (uint64_t e = (uint64_t)e0 + tile; e < e1; e += tile) push_cut(out, (uint32_t)e);
     push_cut(out, e1);
+}
+
+// Column-ordered merge tiles: entry k of tile [j0, j1) becomes the key
+// col << pb | (k - j0); each tile's keys are sorted (column order, ties in row
+// order) and its values permuted to match.  Tiles are independent, so they
+// are spread over host threads.
+void column_ordered_tiles(const sp_csr *A, const std::vector<uint32_t> &cnnz, uint32_t pb, size_t sv,
+                          std::vector<uint32_t> &key, std::vector<char> &val) {
+    key.resize(A->nnz);
+    val.resize((size_t)A->nnz * sv);
+    const size_t nt = cnnz.empty() ? 0 : cnnz.size() - 1;
+    const char *src = (const char *)A->values;
+    auto work = [&](size_t t0, size_t step) {
+        for (size_t t = t0; t < nt; t += step) {
+            const uint32_t j0 = cnnz[t], j1 = cnnz[t + 1];
+            for (uint32_t k = j0; k < j1; k++) key[k] = (A->colind[k] << pb) | (k - j0);
+            std::sort(key.begin() + j0, key.begin() + j1);
+            const uint32_t mask = (1u << pb) - 1;
+            for (uint32_t k = j0; k < j1; k++)
+                memcpy(&val[(size_t)k * sv], src + (size_t)(j0 + (key[k] & mask)) * sv, sv);
+        }
+    };
+    const size_t nth = std::min<size_t>({16, std::max(1u, std::thread::hardware_concurrency()), nt / 64 + 1});
+    std::vector<std::thread> th;
+    for (size_t i = 1; i < nth; i++) th.emplace_back(work, i, nth);
+    work(0, nth);
+    for (auto &t : th) t.join();
 }
 
 bool coo_sorted(const sp_coo *A) {
@@ -184,10 +225,28 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
     void *d;
     if ((rc = plan_upload(p, A->rowptr, ((size_t)A->nrows + 1) * 4, &d))) return rc;
     P.ptr = (const uint32_t *)d;
-    if ((rc = plan_upload(p, A->colind, (size_t)A->nnz * 4, &d))) return rc;
-    P.cind = (const uint32_t *)d;
-    if ((rc = plan_upload(p, A->values, (size_t)A->nnz * sv, &d))) return rc;
-    P.val = d;
+    std::vector<uint32_t> crow, cnnz;
+    if (p->algo == SP_ALGO_SEGMENTED_SORTED) {
+        const uint32_t I = pick_ipt(p->opts, 8);
+        if (!I) return SP_ERR_ARG;
+        uint32_t pb = 0;
+        while ((1u << pb) < TPB * I) pb++;
+        if (A->ncols > (1ull << (32 - pb))) return SP_ERR_UNSUPPORTED;  // col << pb must fit 32 bits
+        if (A->nrows) merge_coords(A->rowptr, A->nrows, A->nnz, TPB * I, crow, cnnz);
+        std::vector<uint32_t> key;
+        std::vector<char> val;
+        column_ordered_tiles(A, cnnz, pb, sv, key, val);
+        if ((rc = plan_upload(p, key.data(), (size_t)A->nnz * 4, &d))) return rc;
+        P.cind = (const uint32_t *)d;
+        if ((rc = plan_upload(p, val.data(), (size_t)A->nnz * sv, &d))) return rc;
+        P.val = d;
+        P.pbits = pb;
+    } else {
+        if ((rc = plan_upload(p, A->colind, (size_t)A->nnz * 4, &d))) return rc;
+        P.cind = (const uint32_t *)d;
+        if ((rc = plan_upload(p, A->values, (size_t)A->nnz * sv, &d))) return rc;
+        P.val = d;
+    }
     P.nrows = P.nrows_elem = A->nrows;
     P.nent = A->nnz;
     P.ncols_elem = A->ncols;
@@ -225,8 +284,7 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
     const uint32_t I = pick_ipt(p->opts, 8);
     if (!I) return SP_ERR_ARG;
     p->ipt = p->work.ipt = I;
-    std::vector<uint32_t> crow, cnnz;
-    if (A->nrows) merge_coords(A->rowptr, A->nrows, A->nnz, TPB * I, crow, cnnz);
+    if (p->algo != SP_ALGO_SEGMENTED_SORTED && A->nrows) merge_coords(A->rowptr, A->nrows, A->nnz, TPB * I, crow, cnnz);
     p->nparts = p->ntiles = crow.empty() ? 0 : (uint32_t)crow.size() - 1;
     p->nsub = TPB;
     b.add(P, crow, &cnnz);
@@ -349,12 +407,15 @@ int build_bcsr_plan(sp_plan *p, const sp_bcsr *B) {
     P.ncols_elem = B->ncols;
     p->algo = SP_ALGO_VECTOR;
     uint32_t L = p->opts.lanes_per_row;
-    if (L == 0) {
-        const double avg = nbr ? (double)B->num_blocks / nbr : 1.0;
+    const double avg = nbr ? (double)B->num_blocks / nbr : 1.0;
+    const bool coal = L == 0 && coal_ok(B->dtype, R, C, avg);
+    if (coal) {
+        L = 64;  // a wave per block row; work.lanes 0 selects the coalesced kernel
+    } else if (L == 0) {
         L = avg >= 48 ? 64 : avg >= 12 ? 16 : avg >= 3 ? 4 : 1;
     }
     if (L != 1 && L != 4 && L != 16 && L != 64) return SP_ERR_ARG;
-    p->lanes = p->work.lanes = L;
+    p->lanes = p->work.lanes = coal ? 0 : L;
     p->work.R = R;
     p->work.C = C;
     const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
@@ -376,7 +437,15 @@ int build_bcsr_plan(sp_plan *p, const sp_bcsr *B) {
                                           p->opts.sub_balance == SP_BAL_NNZ, ts.data())))
             return rc;
         ts[T] = r1;
-        for (uint32_t i = 0; i <= T; i++) push_cut(units, std::min(std::max(ts[i], r0), r1));
+        uint32_t prev = r0;
+        for (uint32_t i = 0; i <= T; i++) {
+            const uint32_t cut = std::min(std::max(ts[i], prev), r1);
+            // cap a unit at 4x the even share (trailing empty block rows land in one tasklet)
+            for (uint64_t c = (uint64_t)prev + 4 * brows_per_unit; c < cut; c += 4 * brows_per_unit)
+                push_cut(units, (uint32_t)c);
+            push_cut(units, cut);
+            prev = cut;
+        }
     }
     push_cut(units, nbr);
     if (nbr == 0) units.clear();
@@ -609,6 +678,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
                 p->algo = (maxlen <= 4 * avg + 64 && A0->nnz <= (16u << 20)) ? SP_ALGO_VECTOR
                                                                              : SP_ALGO_SEGMENTED;
             }
+            if (p->algo == SP_ALGO_SEGMENTED_SORTED) p->algo = SP_ALGO_SEGMENTED;  // 1D CSR only
             kind = p->algo == SP_ALGO_VECTOR ? WK_CSR_VEC : WK_CSR_MERGE;
         } else if (tk.format == SP_FMT_COO) {
             kind = WK_COO_SEG;
@@ -624,11 +694,13 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
         if (!I) { rc = SP_ERR_ARG; break; }
         uint32_t L = p->opts.lanes_per_row;
         if (kind == WK_CSR_VEC && L == 0) L = auto_lanes(n ? (double)A0->nnz / n / vert : 1.0);
+        bool coal = false;
         if (kind == WK_BCSR_VEC && L == 0) {
             const double avg = T->num_block_rows ? (double)T->num_blocks / ((double)T->num_block_rows * nt) : 1.0;
-            L = avg >= 48 ? 64 : avg >= 12 ? 16 : avg >= 3 ? 4 : 1;
+            coal = coal_ok(A0->dtype, R, C, avg);
+            L = coal ? 64 : avg >= 48 ? 64 : avg >= 12 ? 16 : avg >= 3 ? 4 : 1;
         }
-        p->lanes = p->work.lanes = L;
+        p->lanes = p->work.lanes = coal ? 0 : L;
         p->ipt = p->work.ipt = kind == WK_BCOO_SEG ? 4 : I;
         p->work.R = R;
         p->work.C = C;

@@ -47,6 +47,15 @@ static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
             p.o.items_per_lane = I;
             v.push_back(p);
         }
+        // column-ordered tiles where col << log2(256 * I) fits 32 bits
+        for (uint32_t I : {4u, 8u, 16u}) {
+            const uint32_t pb = I == 4 ? 10 : I == 8 ? 11 : 12;
+            if (A->ncols > (1ull << (32 - pb))) continue;
+            Pt p{id, base};
+            p.o.algo = SP_ALGO_SEGMENTED_SORTED;
+            p.o.items_per_lane = I;
+            v.push_back(p);
+        }
     } else if (id >= SP_K_COO_ROW && id <= SP_K_COO_NNZ) {
         for (uint32_t I : {8u, 16u}) {
             Pt p{id, base};

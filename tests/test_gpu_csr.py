@@ -15,7 +15,8 @@ DTYPES = [0, 1, 2, 3, 4, 5]
 ALGOS = [("vec1", dict(algo=1, lanes_per_row=1)), ("vec4", dict(algo=1, lanes_per_row=4)),
          ("vec16", dict(algo=1, lanes_per_row=16)), ("vec64", dict(algo=1, lanes_per_row=64)),
          ("seg4", dict(algo=2, items_per_lane=4)), ("seg8", dict(algo=2, items_per_lane=8)),
-         ("seg16", dict(algo=2, items_per_lane=16))]
+         ("seg16", dict(algo=2, items_per_lane=16)), ("sorted4", dict(algo=3, items_per_lane=4)),
+         ("sorted16", dict(algo=3, items_per_lane=16))]
 
 
 def test_stencil5_int32_bit_exact(sp, ctx):
@@ -83,7 +84,7 @@ def test_spmm(sp, ctx, dt, k):
     A = sp.gen_rmat(11, 20000, dtype=dt, seed=12)
     X = np.stack([sp.gen_vector(dt, A.ncols, seed=20 + q) for q in range(k)], axis=1)
     for name, kw in [("vec4", dict(algo=1, lanes_per_row=4)), ("seg4", dict(algo=2, items_per_lane=4)),
-                     ("seg8", dict(algo=2, items_per_lane=8))]:
+                     ("seg8", dict(algo=2, items_per_lane=8)), ("sorted8", dict(algo=3, items_per_lane=8))]:
         Y, _ = run(sp, ctx, "CSR-nnz", A, X, sp.opts(**kw), k=k)
         assert_parity_multi(dt, Y, A, X, name)
 
@@ -104,3 +105,43 @@ def test_rmat_full_size(sp, ctx, dt):
     x = sp.gen_vector(dt, A.ncols, seed=22)
     y, plan = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=2, items_per_lane=8))
     assert_parity(dt, y, A, x, "rmat-full")
+
+
+@pytest.mark.parametrize("dt", [4, 5])
+@pytest.mark.parametrize("ipt", [4, 8, 16])
+def test_sorted_tiles_same_bits(sp, ctx, dt, ipt):
+    """Column-ordered tiles put the products back in row order before the
+    segmented sums, so every float result is bit-identical to the row-ordered
+    merge kernel with the same tile size (one and several right-hand sides)."""
+    A = sp.gen_rmat(16, 900000, dtype=dt, seed=31)
+    x = sp.gen_vector(dt, A.ncols, seed=32)
+    y0, _ = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=2, items_per_lane=ipt))
+    y1, _ = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=3, items_per_lane=ipt))
+    assert_parity(dt, y1, A, x, "sorted")
+    assert np.array_equal(y0, y1)
+    X = np.stack([sp.gen_vector(dt, A.ncols, seed=40 + q) for q in range(5)], axis=1)
+    Y0, _ = run(sp, ctx, "CSR-nnz", A, X, sp.opts(algo=2, items_per_lane=ipt), k=5)
+    Y1, _ = run(sp, ctx, "CSR-nnz", A, X, sp.opts(algo=3, items_per_lane=ipt), k=5)
+    assert np.array_equal(Y0, Y1)
+
+
+def test_sorted_tiles_column_limit(sp, ctx):
+    """col << log2(tile) must fit 32 bits: 2^20 columns take 16 items per lane,
+    one more column is refused; 8 items per lane take up to 2^21."""
+    for ncols, ipt, ok in ((1 << 20, 16, True), ((1 << 20) + 1, 16, False), ((1 << 20) + 1, 8, True)):
+        A = _coo(sp, 4, ncols, [0, 1, 3, 3], [0, ncols - 1, 5, ncols - 1], sp.I32)
+        x = (np.arange(ncols) % 7 - 3).astype(np.int32)
+        if ok:
+            y, _ = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=3, items_per_lane=ipt))
+            assert_parity(sp.I32, y, A, x, f"ncols {ncols}")
+        else:
+            with pytest.raises(sp._lib.SparsePError):
+                sp.Plan(ctx, "CSR-nnz", A, sp.opts(algo=3, items_per_lane=ipt))
+
+
+def test_rmat_full_size_sorted(sp, ctx):
+    """The bench matrix with column-ordered 16-item tiles (2^20 columns: the largest that fits)."""
+    A = sp.gen_rmat(20, 1 << 24, dtype=sp.F32, seed=2)
+    x = sp.gen_vector(sp.F32, A.ncols, seed=22)
+    y, _ = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=3, items_per_lane=16))
+    assert_parity(sp.F32, y, A, x, "rmat-full-sorted")

@@ -42,9 +42,12 @@ def test_all_1d_variants(sp, ctx, dt):
                 assert_parity(dt, y, A, x, f"{mname}/{kern}/{oname}")
 
 
-@pytest.mark.parametrize("rc", [(1, 1), (2, 2), (8, 8), (2, 4), (4, 2)])
-@pytest.mark.parametrize("dt", [2, 4, 5])
+@pytest.mark.parametrize("rc", [(1, 1), (2, 2), (4, 4), (8, 8), (2, 4), (4, 2)])
+@pytest.mark.parametrize("dt", [1, 2, 3, 4, 5])
 def test_block_sizes(sp, ctx, rc, dt):
+    """Default lanes pick the coalesced BCSR kernel where a block row holds whole
+    16-byte chunks (e.g. FP64 2x2/4x4, FP32 4x4/8x8, INT16 8x8); explicit lane
+    counts pick the register-blocked kernel."""
     r, c = rc
     for A in (sp.gen_rmat(11, 20000, dtype=dt, seed=12), sp.gen_stencil27(5, 4, False, dt, seed=4),
               sp.gen_uniform(1001, 999, 1, 20, dtype=dt, seed=2)):
