@@ -94,7 +94,9 @@ def wl_rmat(sp, ctx, args, orc):
     pinfo = plan.info()
     algo_name = {sp.ALGO_SEGMENTED: f"merge-path, {pinfo['items_per_lane']} items per lane",
                  sp.ALGO_SEGMENTED_SORTED: f"merge-path with column-ordered tiles, {pinfo['items_per_lane']} "
-                                           "items per lane"}.get(pinfo["algo"],
+                                           "items per lane",
+                 sp.ALGO_SEGMENTED_HOT: "merge-path with column-ordered tiles and the most used x entries "
+                                        "staged in LDS"}.get(pinfo["algo"],
                                                                  f"vector, {pinfo['lanes_per_row']} lanes per row")
     log(f"[bench] CSR-nnz configuration ({chosen}): {algo_name}")
     algo_bytes = pinfo["algo_bytes"]
@@ -135,7 +137,8 @@ def wl_rmat(sp, ctx, args, orc):
     # (FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy kernel of the same runs)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "csr_merge_rmat20_f32_traffic.json")
-    tkey = {sp.ALGO_SEGMENTED: "merge", sp.ALGO_SEGMENTED_SORTED: "sorted"}.get(pinfo["algo"])
+    tkey = {sp.ALGO_SEGMENTED: "merge", sp.ALGO_SEGMENTED_SORTED: "sorted",
+            sp.ALGO_SEGMENTED_HOT: "hot"}.get(pinfo["algo"])
     if os.path.exists(tpath) and tkey and args.scale == 20 and args.nnz_log2 == 24:
         with open(tpath) as f:
             t = json.load(f)["per_config"].get(f"{tkey}{pinfo['items_per_lane']}")

@@ -400,7 +400,12 @@ typedef enum sp_algo {
      * lanes of one gather instruction share cache lines; products return to
      * row order in LDS, giving the same sums in the same order.  CSR ids
      * 0 and 1 only; needs ncols <= 2^32 / (256 * items_per_lane). */
-    SP_ALGO_SEGMENTED_SORTED = 3
+    SP_ALGO_SEGMENTED_SORTED = 3,
+    /* SEGMENTED_SORTED with the most frequent columns' x entries staged in
+     * LDS (up to 112 KB; 96 KB for 8-byte accumulators): one 1024-thread
+     * workgroup per CU, tiles of 4096 merge items.  CSR ids 0 and 1 only;
+     * needs ncols <= 2^20.  Several right-hand sides run one at a time. */
+    SP_ALGO_SEGMENTED_HOT = 4
 } sp_algo;
 
 /* Tasklet-level balance (the reference BLNC_TSKLT_* knobs). */

@@ -110,7 +110,7 @@ extern "C" int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, co
     std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, int>, size_t> gidx;
     struct Acc {
         std::vector<spd::Prob> probs;
-        std::vector<uint32_t> unit_pid, c0, c1, zpid, zrow;
+        std::vector<uint32_t> unit_pid, c0, c1, c2, zpid, zrow;
     };
     std::vector<Acc> acc;
     for (int i = 0; i < nprob; i++) {
@@ -154,6 +154,8 @@ extern "C" int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, co
             for (uint32_t u : p->h_unit_pid) a.unit_pid.push_back(u + pid0);
             a.c0.insert(a.c0.end(), p->h_c0.begin(), p->h_c0.end());
             a.c1.insert(a.c1.end(), p->h_c1.begin(), p->h_c1.end());
+            a.c2.insert(a.c2.end(), p->h_c2.begin(), p->h_c2.end());
+            g.w.hcol_max = std::max(g.w.hcol_max, w.hcol_max);
             for (size_t z = 0; z < p->h_zrow.size(); z++) {
                 a.zpid.push_back(p->h_zpid[z] + pid0);
                 a.zrow.push_back(p->h_zrow[z]);
@@ -171,13 +173,23 @@ extern "C" int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, co
         w.nunits = (uint32_t)a.unit_pid.size();
         w.nzero = (uint32_t)a.zrow.size();
         // persistent grid: enough workgroups to fill every CU several times over
-        w.grid_cap = (uint32_t)std::max(1, ctx->num_cus) * 8;
+        w.grid_cap = (uint32_t)std::max(1, ctx->num_cus) * (w.kind == WK_CSR_HOT ? 1 : 8);
+        if (w.kind == WK_CSR_HOT) {
+            // every slot stages its own x (a plan may appear in several slots)
+            for (spd::Prob &P : a.probs) {
+                void *xh = nullptr;
+                if ((rc = balloc(b, ((size_t)P.nhcol * sp_dtype_size(dt) + 15) / 16 * 16, &xh))) break;
+                P.xh = xh;
+            }
+            if (rc) break;
+        }
         if ((rc = bupload(b, a.probs, &w.d_prob))) break;
         if ((rc = bupload(b, a.unit_pid, &w.d_unit_pid))) break;
         if ((rc = bupload(b, a.c0, &w.d_c0))) break;
         if (!a.c1.empty() && (rc = bupload(b, a.c1, &w.d_c1))) break;
+        if (!a.c2.empty() && (rc = bupload(b, a.c2, &w.d_c2))) break;
         if (w.nzero && ((rc = bupload(b, a.zpid, &w.d_zpid)) || (rc = bupload(b, a.zrow, &w.d_zrow)))) break;
-        const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG;
+        const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG || w.kind == WK_CSR_HOT;
         if (carries) {
             const uint32_t S = w.kind == WK_BCOO_SEG ? w.R : 1;
             const size_t n = (size_t)std::max<uint32_t>(w.nunits, 1) * S;

@@ -43,7 +43,9 @@ struct Prob {
     int64_t xoff, yoff;    // element offsets into the per-call x / y bases
     int32_t ldx, ldy;      // leading dimensions when x / y come from pointer tables
     uint32_t pbits;        // CSR merge tiles in column order: cind holds col << pbits | position in tile
-    uint32_t pad;
+    uint32_t nhcol;        // hot-x CSR: number of staged ("hot") columns
+    const uint32_t *hcol;  // hot-x CSR: the staged columns, slot order
+    void *xh;              // hot-x CSR: x at the staged columns, written before each launch
 };
 
 template <typename T> struct Ptrs {
@@ -78,13 +80,13 @@ template <typename A> __device__ __forceinline__ A shfl_up_any(A v, int off) {
     }
 }
 
-// Workgroup-wide inclusive segmented scan of (key, val[K]) over TPB threads;
+// Workgroup-wide inclusive segmented scan of (key, val[K]) over NT threads;
 // keys are non-decreasing in thread order.  On return val holds the scan
 // and s_scan[q][tid] the same values for neighbours to read.
-template <typename A, int K>
+template <typename A, int K, int NT = TPB>
 __device__ __forceinline__ void block_seg_scan(uint32_t key, A (&val)[K], uint32_t *s_key,
-                                               A (*s_scan)[TPB], A (*s_wc)[TPB / WAVE]) {
-    constexpr int NW = TPB / WAVE;
+                                               A (*s_scan)[NT], A (*s_wc)[NT / WAVE]) {
+    constexpr int NW = NT / WAVE;
     const int tid = threadIdx.x, lane = tid % WAVE, wid = tid / WAVE;
 #pragma unroll
     for (int off = 1; off < WAVE; off <<= 1) {
@@ -265,6 +267,182 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
         for (int q = 0; q < K; q++) carry_val[(size_t)t * K + q] = acc[q];
     }
     __syncthreads();  // LDS is reused by the next tile
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR, merge path with the most used x entries staged in LDS.  The plan picks
+// up to Hot<T>::SLOTS "hot" columns (the most frequent ones) and stores every
+// merge tile in column order with the hot entries first: entry k of a tile is
+// hot iff k < chot[tile], and its index word then holds the LDS slot of its
+// column instead of the column.  hot_gather_kernel copies x at the hot
+// columns into a contiguous buffer before each launch; each 1024-thread
+// workgroup (one per CU) copies that buffer into LDS once and then sweeps
+// tiles persistently, so hot gathers are LDS reads and only cold entries
+// gather from L2.  Products go to their row-order slots and the segmented
+// sums run as in csr_merge_kernel.
+// ---------------------------------------------------------------------------
+constexpr int HOT_TPB = 1024;
+constexpr int HOT_IPT = 4;
+template <typename T> struct Hot {
+    // LDS left after the tile buffers (row ends, products, scan) of 4 items per lane
+    static constexpr int BYTES = sizeof(acc_t<T>) == 8 ? 96 * 1024 : 112 * 1024;
+    static constexpr int SLOTS = BYTES / (int)sizeof(T);
+};
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void hot_gather_kernel(const Prob *__restrict__ probs, Ptrs<T> io) {
+    const uint32_t pid = blockIdx.y;
+    const Prob P = probs[pid];
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<const uint32_t> __restrict__ hc = gp(P.hcol);
+    const gptr<T> __restrict__ xh = gp((T *)P.xh);
+    const int ldx = io.lx(P);
+    for (uint32_t s = blockIdx.x * TPB + threadIdx.x; s < P.nhcol; s += gridDim.x * TPB)
+        xh[s] = x[(size_t)ldg(hc + s) * ldx];
+}
+
+template <typename T>
+__global__ __launch_bounds__(HOT_TPB) void csr_hot_kernel(const Prob *__restrict__ probs,
+                                                          const uint32_t *__restrict__ unit_pid,
+                                                          const uint32_t *__restrict__ crow,
+                                                          const uint32_t *__restrict__ cnnz,
+                                                          const uint32_t *__restrict__ chot, Ptrs<T> io,
+                                                          uint32_t *__restrict__ carry_row,
+                                                          acc_t<T> *__restrict__ carry_val, uint32_t nunits) {
+    using A = acc_t<T>;
+    constexpr int NT = HOT_TPB, IPT = HOT_IPT, TILE = NT * IPT;
+    constexpr int H = Hot<T>::SLOTS;
+    __shared__ __align__(16) T s_x[H];
+    __shared__ uint32_t s_rowend[TILE + 1];
+    __shared__ A s_prod[TILE];
+    __shared__ uint32_t s_key[NT];
+    __shared__ A s_scan[1][NT];
+    __shared__ A s_wc[1][NT / WAVE];
+
+    const int tid = threadIdx.x;
+    // One workgroup per CU leaves no other workgroup to hide a tile's load
+    // latency behind, so the next tile's row ends, indices and values are
+    // loaded while the current tile is walked and scanned.
+    struct Tile { uint32_t pid, i0, j0, na, nb, nh, nrl; };
+    auto fetch = [&](uint32_t tt, Tile &ti, uint32_t (&re)[IPT + 1], uint32_t (&cv)[IPT], T (&vv)[IPT]) {
+        ti.pid = unit_pid[tt];
+        const uint32_t ci = tt + ti.pid;
+        const Prob &Q = probs[ti.pid];
+        ti.i0 = crow[ci];
+        ti.j0 = cnnz[ci];
+        ti.nh = chot[ci];
+        ti.na = crow[ci + 1] - ti.i0;
+        ti.nb = cnnz[ci + 1] - ti.j0;
+        ti.nrl = min(ti.na + 1, Q.nrows - ti.i0);
+        const gptr<const uint32_t> __restrict__ rp = gp(Q.ptr), cip = gp(Q.cind);
+        const gptr<const T> __restrict__ vals = gp((const T *)Q.val);
+#pragma unroll
+        for (int q = 0; q <= IPT; q++) {
+            const uint32_t k = tid + q * NT;
+            if (k < ti.nrl) re[q] = rp[ti.i0 + k + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const uint32_t k = tid + q * NT;
+            if (k < ti.nb) {
+                cv[q] = ldg(cip + ti.j0 + k);
+                vv[q] = ldg(vals + ti.j0 + k);
+            }
+        }
+    };
+    Tile cur{};
+    uint32_t re[IPT + 1], cv[IPT];
+    T vv[IPT];
+    if (blockIdx.x < nunits) fetch(blockIdx.x, cur, re, cv, vv);
+    uint32_t staged = NONE;  // problem whose hot x is in LDS
+    for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
+    const uint32_t pid = cur.pid;
+    const Prob P = probs[pid];
+    if (pid != staged) {
+        // the previous tile ended with a barrier, so s_x is free; the copy is
+        // 16-byte chunks of the (padded) staged buffer
+        const gptr<const u32x4> __restrict__ src = (gptr<const u32x4>)gp((const T *)P.xh);
+        const uint32_t nchunk = (P.nhcol * (uint32_t)sizeof(T) + 15) / 16;
+        for (uint32_t c = tid; c < nchunk; c += NT) ((u32x4 *)s_x)[c] = src[c];
+        staged = pid;
+    }
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const int ldx = io.lx(P), ldy = io.ly(P);
+    const uint32_t i0 = cur.i0, j0 = cur.j0, na = cur.na, nb = cur.nb, nh = cur.nh, nrl = cur.nrl;
+    const uint32_t pb = P.pbits, pmask = (1u << pb) - 1;
+
+    // cold gathers first (global), then the staging copy must be visible for the LDS reads
+    T xv[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const uint32_t k = tid + q * NT;
+        if (k < nb && k >= nh) xv[q] = x[(size_t)(cv[q] >> pb) * ldx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const uint32_t k = tid + q * NT;
+        if (k < nh) xv[q] = s_x[cv[q] >> pb];
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; q++)
+        if (tid + q * NT < nb) s_prod[cv[q] & pmask] = mul<T>(vv[q], xv[q]);
+#pragma unroll
+    for (int q = 0; q <= IPT; q++)
+        if (tid + q * NT < nrl) s_rowend[tid + q * NT] = re[q];
+    __syncthreads();
+
+    // the next tile's loads are in flight during the walk and the scan
+    Tile nxt{};
+    uint32_t re2[IPT + 1], cv2[IPT];
+    T vv2[IPT];
+    if (t + gridDim.x < nunits) fetch(t + gridDim.x, nxt, re2, cv2, vv2);
+
+    const uint32_t total = na + nb;
+    const uint32_t d0 = min((uint32_t)tid * IPT, total), d1 = min(d0 + IPT, total);
+    uint32_t li = merge_search(d0, na, nb, [&](uint32_t i) { return s_rowend[i]; }, j0);
+    uint32_t lj = d0 - li;
+    A acc[1] = {A(0)}, first = A(0);
+    bool emitted = false;
+    uint32_t first_row = 0;
+    for (uint32_t s = d0; s < d1; s++) {
+        if (li < na && s_rowend[li] <= j0 + lj) {
+            const uint32_t row = i0 + li;
+            if (!emitted) {
+                emitted = true;
+                first_row = row;
+                first = acc[0];
+            } else {
+                y[(size_t)row * ldy] = to_val<T>(acc[0]);
+            }
+            acc[0] = A(0);
+            li++;
+        } else {
+            acc[0] = acc[0] + s_prod[lj];
+            lj++;
+        }
+    }
+    const uint32_t key = i0 + li;
+    block_seg_scan<A, 1, NT>(key, acc, s_key, s_scan, s_wc);
+    if (emitted) {
+        const A cin = tid > 0 ? s_scan[0][tid - 1] : A(0);
+        y[(size_t)first_row * ldy] = to_val<T>(cin + first);
+    }
+    if (tid == NT - 1) {
+        carry_row[t] = key < P.nrows ? key : NONE;
+        carry_val[t] = acc[0];
+    }
+    __syncthreads();  // LDS tile buffers are reused by the next tile
+    cur = nxt;
+#pragma unroll
+    for (int q = 0; q <= IPT; q++) re[q] = re2[q];
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        cv[q] = cv2[q];
+        vv[q] = vv2[q];
+    }
     }
 }
 

@@ -39,6 +39,18 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         }
         return SP_ERR_UNSUPPORTED;
     } else {
+    if (w.kind == WK_CSR_HOT) {
+        if constexpr (K == 1) {
+            // x at the hot columns for every problem, then one workgroup per CU (grid_cap)
+            const uint32_t gx = std::min<uint32_t>(std::max<uint32_t>((w.hcol_max + TPB - 1) / TPB, 1), 256);
+            hipLaunchKernelGGL((hot_gather_kernel<T>), dim3(gx, w.nprob), dim3(TPB), 0, s, w.d_prob, io);
+            hipLaunchKernelGGL((csr_hot_kernel<T>), grid, dim3(HOT_TPB), 0, s, w.d_prob, w.d_unit_pid, w.d_c0,
+                               w.d_c1, w.d_c2, io, w.d_carry_row, cv, w.nunits);
+            return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+        } else {
+            return SP_ERR_UNSUPPORTED;  // one right-hand side per pass (max_rhs_chunk)
+        }
+    }
     switch (w.kind) {
     case WK_CSR_MERGE:
         switch (w.ipt) {
@@ -115,7 +127,7 @@ static int launch_chunk(const Work &w, sp_ctx *ctx, Ptrs<T> io) {
         if (rc) return rc;
     }
     mark_dominant(ctx);
-    const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG;
+    const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG || w.kind == WK_CSR_HOT;
     const uint32_t S = w.kind == WK_BCOO_SEG ? w.R : 1;
     const uint64_t nthreads = (carries ? (uint64_t)S * w.nunits : 0) + w.nzero;
     if (nthreads) {

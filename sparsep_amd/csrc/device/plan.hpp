@@ -46,7 +46,7 @@ struct sp_plan {
     Work work;                // the main launch group
     // host copies of the work arrays (batches concatenate them)
     std::vector<spd::Prob> h_probs;
-    std::vector<uint32_t> h_unit_pid, h_c0, h_c1, h_zpid, h_zrow;
+    std::vector<uint32_t> h_unit_pid, h_c0, h_c1, h_c2, h_zpid, h_zrow;
     // 2D tilings: the work writes per-stripe partial rows into ypart
     // ([vert][nrows][k]); a reduction adds them into y in stripe order.
     bool tiled = false;

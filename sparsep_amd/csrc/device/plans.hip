@@ -23,12 +23,17 @@ size_t acc_size(int dt) { return (dt == SP_I64 || dt == SP_F64) ? 8 : 4; }
 struct Builder {
     sp_plan *p;
     std::vector<Prob> probs;
-    std::vector<uint32_t> unit_pid, c0, c1;
+    std::vector<uint32_t> unit_pid, c0, c1, c2;
     std::vector<uint32_t> zpid, zrow;
 
     explicit Builder(sp_plan *pl) : p(pl) {}
 
-    void add(const Prob &P, const std::vector<uint32_t> &k0, const std::vector<uint32_t> *k1) {
+    void add(const Prob &P, const std::vector<uint32_t> &k0, const std::vector<uint32_t> *k1,
+             const std::vector<uint32_t> *k2 = nullptr) {
+        if (k2) {
+            if (k2->empty()) c2.push_back(0);
+            else c2.insert(c2.end(), k2->begin(), k2->end());
+        }
         const uint32_t pid = (uint32_t)probs.size();
         probs.push_back(P);
         const size_t nu = k0.empty() ? 0 : k0.size() - 1;
@@ -52,6 +57,7 @@ struct Builder {
         if ((rc = plan_upload(p, unit_pid.data(), unit_pid.size() * 4, (void **)&w.d_unit_pid))) return rc;
         if ((rc = plan_upload(p, c0.data(), c0.size() * 4, (void **)&w.d_c0))) return rc;
         if (!c1.empty() && (rc = plan_upload(p, c1.data(), c1.size() * 4, (void **)&w.d_c1))) return rc;
+        if (!c2.empty() && (rc = plan_upload(p, c2.data(), c2.size() * 4, (void **)&w.d_c2))) return rc;
         w.nzero = (uint32_t)zrow.size();
         if (w.nzero) {
             if ((rc = plan_upload(p, zpid.data(), zpid.size() * 4, (void **)&w.d_zpid))) return rc;
@@ -68,6 +74,7 @@ struct Builder {
         p->h_unit_pid = unit_pid;
         p->h_c0 = c0;
         p->h_c1 = c1;
+        p->h_c2 = c2;
         p->h_zpid = zpid;
         p->h_zrow = zrow;
         return SP_OK;
@@ -190,6 +197,74 @@ void column_ordered_tiles(const sp_csr *A, const std::vector<uint32_t> &cnnz, ui
     for (auto &t : th) t.join();
 }
 
+// Staged-x slots of the hot-x kernel for a value type.
+uint32_t hot_slots(int dt) {
+    switch (dt) {
+    case SP_I8: return spd::Hot<int8_t>::SLOTS;
+    case SP_I16: return spd::Hot<int16_t>::SLOTS;
+    case SP_I32: return spd::Hot<int32_t>::SLOTS;
+    case SP_I64: return spd::Hot<int64_t>::SLOTS;
+    case SP_F32: return spd::Hot<float>::SLOTS;
+    default: return spd::Hot<double>::SLOTS;
+    }
+}
+
+// Hot-x tiles: the staged columns are the up-to-`slots` most frequent ones
+// used at least twice (ties to the lower column), numbered in column order.
+// Each merge tile stores its staged entries first, ordered by slot, then the
+// others by column; the key is id << pb | position in the tile, with id the
+// slot or the column.  nhot[t] counts tile t's staged entries.
+void hot_tiles(const sp_csr *A, const std::vector<uint32_t> &cnnz, uint32_t pb, size_t sv, uint32_t slots,
+               std::vector<uint32_t> &hcol, std::vector<uint32_t> &nhot, std::vector<uint32_t> &key,
+               std::vector<char> &val) {
+    std::vector<uint32_t> cnt(A->ncols, 0);
+    for (uint32_t k = 0; k < A->nnz; k++) cnt[A->colind[k]]++;
+    std::vector<uint32_t> cols;
+    for (uint32_t c = 0; c < A->ncols; c++)
+        if (cnt[c] >= 2) cols.push_back(c);
+    if (cols.size() > slots) {
+        std::nth_element(cols.begin(), cols.begin() + slots, cols.end(), [&](uint32_t a, uint32_t b) {
+            return cnt[a] != cnt[b] ? cnt[a] > cnt[b] : a < b;
+        });
+        cols.resize(slots);
+    }
+    std::sort(cols.begin(), cols.end());
+    hcol = cols;
+    std::vector<uint32_t> slot(A->ncols, spd::NONE);
+    for (uint32_t s = 0; s < (uint32_t)hcol.size(); s++) slot[hcol[s]] = s;
+    key.resize(A->nnz);
+    val.resize((size_t)A->nnz * sv);
+    const size_t nt = cnnz.empty() ? 0 : cnnz.size() - 1;
+    nhot.assign(nt + 1, 0);
+    const char *src = (const char *)A->values;
+    const uint32_t mask = (1u << pb) - 1;
+    auto work = [&](size_t t0, size_t step) {
+        std::vector<uint32_t> hot, cold;
+        for (size_t t = t0; t < nt; t += step) {
+            const uint32_t j0 = cnnz[t], j1 = cnnz[t + 1];
+            hot.clear();
+            cold.clear();
+            for (uint32_t k = j0; k < j1; k++) {
+                const uint32_t c = A->colind[k], sl = slot[c];
+                if (sl != spd::NONE) hot.push_back((sl << pb) | (k - j0));
+                else cold.push_back((c << pb) | (k - j0));
+            }
+            std::sort(hot.begin(), hot.end());
+            std::sort(cold.begin(), cold.end());
+            nhot[t] = (uint32_t)hot.size();
+            std::copy(hot.begin(), hot.end(), key.begin() + j0);
+            std::copy(cold.begin(), cold.end(), key.begin() + j0 + hot.size());
+            for (uint32_t k = j0; k < j1; k++)
+                memcpy(&val[(size_t)k * sv], src + (size_t)(j0 + (key[k] & mask)) * sv, sv);
+        }
+    };
+    const size_t nth = std::min<size_t>({16, std::max(1u, std::thread::hardware_concurrency()), nt / 64 + 1});
+    std::vector<std::thread> th;
+    for (size_t i = 1; i < nth; i++) th.emplace_back(work, i, nth);
+    work(0, nth);
+    for (auto &t : th) t.join();
+}
+
 bool coo_sorted(const sp_coo *A) {
     for (uint32_t k = 1; k < A->nnz; k++) {
         if (A->rowind[k] < A->rowind[k - 1]) return false;
@@ -226,6 +301,37 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
     if ((rc = plan_upload(p, A->rowptr, ((size_t)A->nrows + 1) * 4, &d))) return rc;
     P.ptr = (const uint32_t *)d;
     std::vector<uint32_t> crow, cnnz;
+    if (p->algo == SP_ALGO_SEGMENTED_HOT) {
+        const uint32_t tile = spd::HOT_TPB * spd::HOT_IPT;
+        uint32_t pb = 0;
+        while ((1u << pb) < tile) pb++;
+        if (A->ncols > (1ull << (32 - pb))) return SP_ERR_UNSUPPORTED;  // col << pb must fit 32 bits
+        if (A->nrows) merge_coords(A->rowptr, A->nrows, A->nnz, tile, crow, cnnz);
+        std::vector<uint32_t> hcol, nhot, key;
+        std::vector<char> val;
+        hot_tiles(A, cnnz, pb, sv, hot_slots(A->dtype), hcol, nhot, key, val);
+        if ((rc = plan_upload(p, key.data(), (size_t)A->nnz * 4, &d))) return rc;
+        P.cind = (const uint32_t *)d;
+        if ((rc = plan_upload(p, val.data(), (size_t)A->nnz * sv, &d))) return rc;
+        P.val = d;
+        if ((rc = plan_upload(p, hcol.data(), hcol.size() * 4, &d))) return rc;
+        P.hcol = (const uint32_t *)d;
+        if ((rc = plan_alloc(p, (hcol.size() * sv + 15) / 16 * 16, &d))) return rc;
+        P.xh = d;
+        P.nhcol = (uint32_t)hcol.size();
+        P.pbits = pb;
+        P.nrows = P.nrows_elem = A->nrows;
+        P.nent = A->nnz;
+        P.ncols_elem = A->ncols;
+        p->ipt = p->work.ipt = spd::HOT_IPT;
+        p->nparts = p->ntiles = crow.empty() ? 0 : (uint32_t)crow.size() - 1;
+        p->nsub = spd::HOT_TPB;
+        p->work.hcol_max = P.nhcol;
+        p->work.grid_cap = (uint32_t)std::max(1, p->ctx->num_cus);  // one workgroup per CU
+        Builder b(p);
+        b.add(P, crow, &cnnz, &nhot);
+        return b.finish(WK_CSR_HOT, true, 1);
+    }
     if (p->algo == SP_ALGO_SEGMENTED_SORTED) {
         const uint32_t I = pick_ipt(p->opts, 8);
         if (!I) return SP_ERR_ARG;
@@ -678,7 +784,8 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
                 p->algo = (maxlen <= 4 * avg + 64 && A0->nnz <= (16u << 20)) ? SP_ALGO_VECTOR
                                                                              : SP_ALGO_SEGMENTED;
             }
-            if (p->algo == SP_ALGO_SEGMENTED_SORTED) p->algo = SP_ALGO_SEGMENTED;  // 1D CSR only
+            if (p->algo == SP_ALGO_SEGMENTED_SORTED || p->algo == SP_ALGO_SEGMENTED_HOT)
+                p->algo = SP_ALGO_SEGMENTED;  // 1D CSR only
             kind = p->algo == SP_ALGO_VECTOR ? WK_CSR_VEC : WK_CSR_MERGE;
         } else if (tk.format == SP_FMT_COO) {
             kind = WK_COO_SEG;

@@ -47,6 +47,11 @@ static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
             p.o.items_per_lane = I;
             v.push_back(p);
         }
+        if (A->ncols <= (1u << 20)) {  // hot-x tiles (4096 items, col << 12 fits 32 bits)
+            Pt p{id, base};
+            p.o.algo = SP_ALGO_SEGMENTED_HOT;
+            v.push_back(p);
+        }
         // column-ordered tiles where col << log2(256 * I) fits 32 bits
         for (uint32_t I : {4u, 8u, 16u}) {
             const uint32_t pb = I == 4 ? 10 : I == 8 ? 11 : 12;

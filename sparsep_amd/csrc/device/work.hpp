@@ -11,7 +11,7 @@
 
 struct sp_ctx;
 
-enum WorkKind { WK_CSR_MERGE = 0, WK_CSR_VEC = 1, WK_COO_SEG = 2, WK_BCSR_VEC = 3, WK_BCOO_SEG = 4 };
+enum WorkKind { WK_CSR_MERGE = 0, WK_CSR_VEC = 1, WK_COO_SEG = 2, WK_BCSR_VEC = 3, WK_BCOO_SEG = 4, WK_CSR_HOT = 5 };
 
 struct Work {
     int kind = WK_CSR_MERGE;
@@ -20,6 +20,8 @@ struct Work {
     spd::Prob *d_prob = nullptr;
     uint32_t *d_unit_pid = nullptr;
     uint32_t *d_c0 = nullptr, *d_c1 = nullptr;  // coordinates, index unit + pid
+    uint32_t *d_c2 = nullptr;                   // hot-x CSR: staged entries per tile, index unit + pid
+    uint32_t hcol_max = 0;                      // hot-x CSR: largest staged-column count of the problems
     uint32_t ipt = 8, lanes = 1, R = 1, C = 1;
     uint32_t *d_carry_row = nullptr;
     void *d_carry_val = nullptr;
@@ -39,6 +41,7 @@ inline int max_rhs_chunk(const Work &w) {
     case WK_CSR_VEC: return 32;
     case WK_BCSR_VEC: return w.R <= 2 ? 32 : w.R <= 4 ? 16 : 8;
     case WK_BCOO_SEG: return 1;
+    case WK_CSR_HOT: return 1;
     default: return 8;
     }
 }

@@ -56,3 +56,27 @@ def test_batch_rejects_mixed_dtypes(sp, ctx):
     B = sp.gen_uniform(100, 100, 1, 5, dtype=5, seed=1)
     with pytest.raises(sp._lib.SparsePError):
         sp.Batch(ctx, [sp.Plan(ctx, "CSR-nnz", A), sp.Plan(ctx, "CSR-nnz", B)], [1, 1])
+
+
+@pytest.mark.parametrize("dt", [2, 4])
+def test_batch_hot_plans(sp, ctx, dt):
+    """Hot-x CSR plans in a batch (one plan in two slots: each slot stages its
+    own x), mixed with merge-path plans, over graph replays."""
+    probs = problems(sp, dt, n=6, seed=9)
+    plans = [sp.Plan(ctx, "CSR-nnz", A, sp.opts(algo=4 if i % 2 == 0 else 2)) for i, (A, k) in enumerate(probs)]
+    plans.append(plans[0])
+    probs.append((probs[0][0], 3))
+    b = sp.Batch(ctx, plans, [k for _, k in probs], graph=True)
+    for rep in range(3):
+        Xs, Xd, Yd = [], [], []
+        for i, (A, k) in enumerate(probs):
+            X = np.stack([sp.gen_vector(dt, A.ncols, seed=300 * rep + 11 * i + q) for q in range(k)], axis=1)
+            Xs.append(X)
+            Xd.append(sp.DeviceArray.from_numpy(ctx, X))
+            y = sp.DeviceArray(ctx, (A.nrows, k), X.dtype)
+            y.fill_bytes(0x5A)
+            Yd.append(y)
+        b.run([x.addr for x in Xd], [y.addr for y in Yd])
+        ctx.sync()
+        for i, (A, k) in enumerate(probs):
+            assert_parity_multi(dt, Yd[i].numpy(), A, Xs[i], f"rep{rep} prob{i}")

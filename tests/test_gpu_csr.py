@@ -16,7 +16,7 @@ ALGOS = [("vec1", dict(algo=1, lanes_per_row=1)), ("vec4", dict(algo=1, lanes_pe
          ("vec16", dict(algo=1, lanes_per_row=16)), ("vec64", dict(algo=1, lanes_per_row=64)),
          ("seg4", dict(algo=2, items_per_lane=4)), ("seg8", dict(algo=2, items_per_lane=8)),
          ("seg16", dict(algo=2, items_per_lane=16)), ("sorted4", dict(algo=3, items_per_lane=4)),
-         ("sorted16", dict(algo=3, items_per_lane=16))]
+         ("sorted16", dict(algo=3, items_per_lane=16)), ("hot", dict(algo=4))]
 
 
 def test_stencil5_int32_bit_exact(sp, ctx):
@@ -84,7 +84,8 @@ def test_spmm(sp, ctx, dt, k):
     A = sp.gen_rmat(11, 20000, dtype=dt, seed=12)
     X = np.stack([sp.gen_vector(dt, A.ncols, seed=20 + q) for q in range(k)], axis=1)
     for name, kw in [("vec4", dict(algo=1, lanes_per_row=4)), ("seg4", dict(algo=2, items_per_lane=4)),
-                     ("seg8", dict(algo=2, items_per_lane=8)), ("sorted8", dict(algo=3, items_per_lane=8))]:
+                     ("seg8", dict(algo=2, items_per_lane=8)), ("sorted8", dict(algo=3, items_per_lane=8)),
+                     ("hot", dict(algo=4))]:
         Y, _ = run(sp, ctx, "CSR-nnz", A, X, sp.opts(**kw), k=k)
         assert_parity_multi(dt, Y, A, X, name)
 
@@ -145,3 +146,34 @@ def test_rmat_full_size_sorted(sp, ctx):
     x = sp.gen_vector(sp.F32, A.ncols, seed=22)
     y, _ = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=3, items_per_lane=16))
     assert_parity(sp.F32, y, A, x, "rmat-full-sorted")
+
+
+@pytest.mark.parametrize("dt", [0, 2, 3, 4, 5])
+def test_hot_staging(sp, ctx, dt):
+    """Hot-x tiles: more frequent columns than LDS slots (the excess stays in
+    global memory), every column hot (a narrow matrix), and no column used
+    twice (nothing staged)."""
+    cases = [sp.gen_rmat(17, 1500000, dtype=dt, seed=41),          # 128K columns, power law
+             sp.gen_uniform(40000, 3000, 10, 60, dtype=dt, seed=42),  # 3000 columns, all hot
+             _coo(sp, 5000, 5000, list(range(5000)), list(range(5000))[::-1], dt)]  # permutation
+    for i, A in enumerate(cases):
+        x = sp.gen_vector(dt, A.ncols, seed=43 + i)
+        y, plan = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=4))
+        assert plan.info()["algo"] == 4
+        assert_parity(dt, y, A, x, f"hot case{i}")
+        y2, _ = run(sp, ctx, "CSR-nnz", A, x, plan=plan)  # the staged buffer is rewritten per call
+        assert np.array_equal(y, y2)
+
+
+def test_hot_column_limit(sp, ctx):
+    A = _coo(sp, 4, (1 << 20) + 1, [0, 1, 3, 3], [0, 1 << 20, 5, 1 << 20], sp.I32)
+    with pytest.raises(sp._lib.SparsePError):
+        sp.Plan(ctx, "CSR-nnz", A, sp.opts(algo=4))
+
+
+def test_rmat_full_size_hot(sp, ctx):
+    """The bench matrix with hot-x tiles (28,672 staged FP32 columns)."""
+    A = sp.gen_rmat(20, 1 << 24, dtype=sp.F32, seed=2)
+    x = sp.gen_vector(sp.F32, A.ncols, seed=22)
+    y, _ = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=4))
+    assert_parity(sp.F32, y, A, x, "rmat-full-hot")
