@@ -485,9 +485,50 @@ def wl_selector(sp, ctx, args, orc):
                         "overhead_median": float(np.median(np.array(sel_ms) / np.array(tune_ms)))})]
 
 
+def wl_spmm(sp, ctx, args, orc):
+    """Several right-hand sides: Y = A X on the 27-point stencil (64^3 x 4 DOF), CSR and BCSR 4 x 4,
+    FP32 and FP64, k in {8, 32}.  Algorithmic bytes: the matrix once plus k(m+n)s."""
+    import numpy as np
+    out = []
+    for dt in [d for d in args.dtypes if d in (sp.F32, sp.F64)]:
+        A = sp.gen_stencil27(64, 4, True, dt, seed=3)
+        s = 4 if dt == sp.F32 else 8
+        for k in (8, 32):
+            X = np.stack([sp.gen_vector(dt, A.ncols, d=sp.dist(sp.DIST_NORMAL, 0, 1), seed=70 + q)
+                          for q in range(k)], axis=1)
+            Xd = sp.DeviceArray.from_numpy(ctx, X)
+            Yd = sp.DeviceArray(ctx, (A.nrows, k), X.dtype)
+            # CSR: the automatic choice (column-ordered merge tiles, 8 columns per pass) and the
+            # row-granular plan (lanes across up to 32 columns)
+            for kname, be, algo in (("CSR-nnz", 2, 0), ("CSR-nnz", 2, 1), ("BCSR-block", 4, 0)):
+                plan = sp.Plan(ctx, kname, A, sp.opts(row_block_size=be, col_block_size=be, algo=algo))
+                info = plan.info()
+                if args.warmup > 0:
+                    plan.time_spmm(k, Xd.addr, k, Yd.addr, k, args.warmup, args.flush_mb << 20)
+                ms = statistics.mean(plan.time_spmm(k, Xd.addr, k, Yd.addr, k, args.steps, args.flush_mb << 20))
+                Y = Yd.numpy()
+                ok = all(parity_ok(orc, dt, A, np.ascontiguousarray(X[:, q]), np.ascontiguousarray(Y[:, q]))
+                         for q in (0, k - 1))
+                algo = info["algo_bytes"] + (k - 1) * (A.nrows + A.ncols) * s
+                gbs = algo / (ms * 1e-3) / 1e9
+                tag = f"{kname} {be}x{be}" if kname.startswith("B") else kname + (" rows" if algo == 1 else "")
+                log(f"[bench] spmm {DT_NAME[dt]} k={k:2d} {tag:14s} {ms * 1e3:8.1f} us {gbs:7.0f} GB/s "
+                    f"{2 * A.nnz * k / (ms * 1e-3) / 1e12:6.2f} TFLOP/s parity(first,last column) {ok}")
+                cfg = {"workload": f"64^3 x 4 DOF 27-point stencil, Y = A X with {k} right-hand sides, {tag}",
+                       "kernel": kname, "k": k, "nnz": A.nnz, "algo_bytes": algo}
+                out.append(line(args, gbs, ms, DT_NAME[dt], cfg,
+                                roofline={"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                                extra={"parity_vs_oracle": ok,
+                                       "tflops": round(2 * A.nnz * k / (ms * 1e-3) / 1e12, 3)}))
+                del plan
+            del Xd, Yd
+    return out
+
+
 WORKLOADS = {"rmat": wl_rmat, "stencil5": wl_stencil5, "formats": wl_formats, "stencil27": wl_stencil27,
              "rmat2d": wl_rmat2d, "phases": wl_phases, "all25": wl_all25,
-             "batched": wl_batched, "selector": wl_selector}
+             "batched": wl_batched, "selector": wl_selector, "spmm": wl_spmm}
 
 
 def main():

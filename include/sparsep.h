@@ -399,7 +399,8 @@ typedef enum sp_algo {
     /* SEGMENTED with every tile's entries stored in column order, so the
      * lanes of one gather instruction share cache lines; products return to
      * row order in LDS, giving the same sums in the same order.  CSR ids
-     * 0 and 1 only; needs ncols <= 2^32 / (256 * items_per_lane). */
+     * 0 and 1 only; needs ncols <= 2^32 / (256 * items_per_lane).  AUTO
+     * takes it in place of SEGMENTED wherever the columns fit. */
     SP_ALGO_SEGMENTED_SORTED = 3,
     /* SEGMENTED_SORTED with the most frequent columns' x entries staged in
      * LDS (up to 112 KB; 96 KB for 8-byte accumulators): one 1024-thread

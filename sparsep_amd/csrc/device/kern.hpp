@@ -246,8 +246,10 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
             } else {
                 const uint32_t c = s_col[lj];
                 const T v = s_val[lj];
+                T xs[K];
+                ld_n<T, K>(x + (size_t)c * ldx, xs);  // the K values of x's row c, wide when aligned
 #pragma unroll
-                for (int q = 0; q < K; q++) acc[q] = acc[q] + mul<T>(v, x[(size_t)c * ldx + q]);
+                for (int q = 0; q < K; q++) acc[q] = acc[q] + mul<T>(v, xs[q]);
             }
             lj++;
         }
@@ -491,17 +493,22 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
         for (; k + L < e; k += 2 * L) {
             const uint32_t c0 = ldg(colind + k), c1 = ldg(colind + k + L);
             const T v0 = ldg(vals + k), v1 = ldg(vals + k + L);
+            T x0[K], x1[K];  // rows c0 and c1 of x, wide loads when aligned
+            ld_n<T, K>(x + (size_t)c0 * ldx, x0);
+            ld_n<T, K>(x + (size_t)c1 * ldx, x1);
 #pragma unroll
             for (int q = 0; q < K; q++) {
-                acc[q] = fma_acc<T>(v0, x[(size_t)c0 * ldx + q], acc[q]);
-                acc[q] = fma_acc<T>(v1, x[(size_t)c1 * ldx + q], acc[q]);
+                acc[q] = fma_acc<T>(v0, x0[q], acc[q]);
+                acc[q] = fma_acc<T>(v1, x1[q], acc[q]);
             }
         }
         if (k < e) {
             const uint32_t c0 = ldg(colind + k);
             const T v0 = ldg(vals + k);
+            T x0[K];
+            ld_n<T, K>(x + (size_t)c0 * ldx, x0);
 #pragma unroll
-            for (int q = 0; q < K; q++) acc[q] = fma_acc<T>(v0, x[(size_t)c0 * ldx + q], acc[q]);
+            for (int q = 0; q < K; q++) acc[q] = fma_acc<T>(v0, x0[q], acc[q]);
         }
 #pragma unroll
         for (int q = 0; q < K; q++) acc[q] = group_sum<L>(acc[q]);
@@ -607,8 +614,10 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
         } else {
             const uint32_t c = s_col[e];
             const T v = s_val[e];
+            T xs[K];
+            ld_n<T, K>(x + (size_t)c * ldx, xs);
 #pragma unroll
-            for (int q = 0; q < K; q++) acc[q] = acc[q] + mul<T>(v, x[(size_t)c * ldx + q]);
+            for (int q = 0; q < K; q++) acc[q] = acc[q] + mul<T>(v, xs[q]);
         }
     }
     if (a < b) {
@@ -695,11 +704,12 @@ __global__ __launch_bounds__(TPB) void bcsr_vector_kernel(const Prob *__restrict
             for (int c = 0; c < C; c++) {
                 const uint32_t col = J * C + c;
                 if (col < ncols) {
+                    T xs[K];  // row col of x, wide loads when aligned
+                    ld_n<T, K>(x + (size_t)col * ldx, xs);
 #pragma unroll
                     for (int q = 0; q < K; q++) {
-                        const T xv = x[(size_t)col * ldx + q];
 #pragma unroll
-                        for (int i = 0; i < R; i++) acc[i][q] = fma_acc<T>(v[i * C + c], xv, acc[i][q]);
+                        for (int i = 0; i < R; i++) acc[i][q] = fma_acc<T>(v[i * C + c], xs[q], acc[i][q]);
                     }
                 }
             }
@@ -773,9 +783,9 @@ __global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__
     for (uint32_t I = r0 + wid; I < r1; I += NW) {
         const uint32_t b = rp[I], e = rp[I + 1];
         A acc = A(0);
-        // U wave steps per iteration with all loads issued first; lanes past
-        // the row's end reload its last block and drop the product
-        constexpr int U = 4;
+        // U wave steps (about 32 blocks) per iteration with all loads issued
+        // first; lanes past the row's end reload its last block and drop the product
+        constexpr int U = CO::PB >= 32 ? 1 : 32 / CO::PB > 8 ? 8 : 32 / CO::PB;
         for (uint32_t k = b + blk_off; k < e; k += U * CO::PB) {
             uint32_t J[U];
             T v[U][CO::CH], xs[U][CO::CH];
@@ -807,6 +817,81 @@ __global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__
         if (blk_off == 0 && (g % CO::CPR) == 0) {
             const uint32_t row = I * R + i;
             if (row < P.nrows_elem) y[(size_t)row * ldy] = to_val<T>(acc);
+        }
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Several right-hand sides (K = 8, 16, 32), CSR and BCSR with R*C <= 16: a
+// wave takes one (block) row at a time, its lanes split into 64/K groups of
+// K, lane q of a group holds right-hand side q.  A group takes every
+// (64/K)-th entry or block of the row: the K lanes read the same values (one
+// transaction) and K consecutive x values of each column (coalesced), so
+// each lane keeps only R sums, and one butterfly over the groups ends the
+// row.  The matrix is read once for all K columns.
+// ---------------------------------------------------------------------------
+template <typename T, int R, int C, int K>
+__global__ __launch_bounds__(TPB) void rows_rhs_kernel(const Prob *__restrict__ probs,
+                                                       const uint32_t *__restrict__ unit_pid,
+                                                       const uint32_t *__restrict__ urow, Ptrs<T> io,
+                                                       uint32_t nunits) {
+    using A = acc_t<T>;
+    static_assert(K == 8 || K == 16 || K == 32, "lanes across 8, 16 or 32 right-hand sides");
+    constexpr int NW = TPB / WAVE, GB = WAVE / K, U = 2;
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    const int q = lane % K, gb = lane / K;
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint32_t pid = unit_pid[u];
+    const Prob P = probs[pid];
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), cip = gp(P.cind);
+    const gptr<const T> __restrict__ bv = gp((const T *)P.val);
+    const int ldx = io.lx(P), ldy = io.ly(P);
+    const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    const uint32_t ncols = P.ncols_elem;
+    for (uint32_t I = r0 + wid; I < r1; I += NW) {
+        const uint32_t b = rp[I], e = rp[I + 1];
+        A acc[R];
+#pragma unroll
+        for (int i = 0; i < R; i++) acc[i] = A(0);
+        // U group steps per iteration, loads first; lanes past the row's end
+        // reload its last entry and drop the product
+        for (uint32_t k = b + gb; k < e; k += U * GB) {
+            uint32_t J[U];
+            T v[U][R * C];
+#pragma unroll
+            for (int w = 0; w < U; w++) {
+                const uint32_t kk = min(k + w * GB, e - 1);
+                J[w] = ldg(cip + kk);
+                ldg_n<T, R * C>(bv + (size_t)kk * (R * C), v[w]);
+            }
+            T xs[U][C];
+#pragma unroll
+            for (int w = 0; w < U; w++)
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const uint32_t col = J[w] * C + c;
+                    xs[w][c] = col < ncols ? x[(size_t)col * ldx + q] : T(0);
+                }
+#pragma unroll
+            for (int w = 0; w < U; w++)
+                if (k + w * GB < e) {
+#pragma unroll
+                    for (int c = 0; c < C; c++)
+#pragma unroll
+                        for (int i = 0; i < R; i++) acc[i] = fma_acc<T>(v[w][i * C + c], xs[w][c], acc[i]);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < R; i++) acc[i] = xor_sum<K, WAVE>(acc[i]);
+        if (gb == 0) {
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const uint32_t row = I * R + i;
+                if (row < P.nrows_elem) y[(size_t)row * ldy + q] = to_val<T>(acc[i]);
+            }
         }
     }
     }

@@ -27,15 +27,25 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
             return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
         }
         if (w.kind == WK_BCSR_VEC) {
-            const uint32_t lanes = w.lanes ? w.lanes : 16;  // coalesced plans: 16 lanes for several RHS
+            if constexpr (sizeof(T) == 8) {
+                // 8-byte values: lanes across the right-hand sides (R sums per lane) beat R*K
+                // register-blocked sums (stencil FP64 4x4, 32 columns: 1.57 vs 2.20 ms)
+#define SPD_RR(Rv, Cv) \
+    if (w.R == Rv && w.C == Cv) { hipLaunchKernelGGL((rows_rhs_kernel<T, Rv, Cv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP; }
+                SPD_RR(1, 1) SPD_RR(2, 2) SPD_RR(2, 4) SPD_RR(4, 2) SPD_RR(4, 4)
+#undef SPD_RR
+                return SP_ERR_UNSUPPORTED;
+            } else {
+                const uint32_t lanes = w.lanes ? w.lanes : 16;  // coalesced plans: 16 lanes for several RHS
 #define SPD_BLW(Rv, Cv, Lv) \
     if (w.R == Rv && w.C == Cv && lanes == Lv) { hipLaunchKernelGGL((bcsr_vector_kernel<T, Rv, Cv, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP; }
 #define SPD_BRCW(Rv, Cv) SPD_BLW(Rv, Cv, 1) SPD_BLW(Rv, Cv, 4) SPD_BLW(Rv, Cv, 16) SPD_BLW(Rv, Cv, 64)
-            if constexpr (K <= 32) { SPD_BRCW(1, 1) SPD_BRCW(2, 2) SPD_BRCW(2, 4) }
-            if constexpr (K <= 16) { SPD_BRCW(4, 4) SPD_BRCW(4, 2) }
+                if constexpr (K <= 32) { SPD_BRCW(1, 1) SPD_BRCW(2, 2) SPD_BRCW(2, 4) }
+                if constexpr (K <= 16) { SPD_BRCW(4, 4) SPD_BRCW(4, 2) }
 #undef SPD_BRCW
 #undef SPD_BLW
-            return SP_ERR_UNSUPPORTED;
+                return SP_ERR_UNSUPPORTED;
+            }
         }
         return SP_ERR_UNSUPPORTED;
     } else {

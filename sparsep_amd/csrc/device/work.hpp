@@ -34,12 +34,15 @@ struct Work {
 };
 
 // Widest right-hand-side chunk a kernel kind takes in one pass: the
-// row-granular kernels keep K (CSR) or R*K (BCSR, R*K <= 64) accumulators per
-// lane; the kernels with carries keep K partial sums per carry slot.
+// row-granular CSR kernel keeps K accumulators per lane; the register-blocked
+// BCSR kernel R*K (R*K <= 64); for 8-byte values and R*C <= 16, lanes run
+// across up to 32 right-hand sides with R sums each; the kernels with carries
+// keep K partial sums per carry slot.
 inline int max_rhs_chunk(const Work &w) {
+    const bool wide8 = (w.dtype == SP_I64 || w.dtype == SP_F64) && w.R * w.C <= 16;
     switch (w.kind) {
     case WK_CSR_VEC: return 32;
-    case WK_BCSR_VEC: return w.R <= 2 ? 32 : w.R <= 4 ? 16 : 8;
+    case WK_BCSR_VEC: return wide8 ? 32 : w.R <= 2 ? 32 : w.R <= 4 ? 16 : 8;
     case WK_BCOO_SEG: return 1;
     case WK_CSR_HOT: return 1;
     default: return 8;

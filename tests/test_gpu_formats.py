@@ -63,7 +63,8 @@ def test_block_sizes(sp, ctx, rc, dt):
 @pytest.mark.parametrize("rc", [(1, 1), (2, 2), (2, 4), (4, 2), (4, 4), (8, 8)])
 @pytest.mark.parametrize("k", [16, 32, 37])
 def test_block_spmm_wide(sp, ctx, rc, k):
-    """Right-hand-side chunks of 16/32 in the register-blocked BCSR kernel (R*K <= 64)."""
+    """Right-hand-side chunks of 8/16/32 with lanes across the right-hand sides
+    (R*C <= 16), and chunks of 8 in the register-blocked kernel (8x8 blocks)."""
     r, c = rc
     A = sp.gen_stencil27(6, 2, False, sp.F32, seed=5)
     X = np.stack([sp.gen_vector(sp.F32, A.ncols, seed=60 + q) for q in range(k)], axis=1)
