@@ -398,9 +398,10 @@ typedef enum sp_algo {
     SP_ALGO_SEGMENTED = 2, /* entry balanced: fixed entry tiles, rows split */
     /* SEGMENTED with every tile's entries stored in column order, so the
      * lanes of one gather instruction share cache lines; products return to
-     * row order in LDS, giving the same sums in the same order.  CSR ids
-     * 0 and 1 only; needs ncols <= 2^32 / (256 * items_per_lane).  AUTO
-     * takes it in place of SEGMENTED wherever the columns fit. */
+     * row order in LDS, giving the same sums in the same order.  1D CSR
+     * and COO ids (0-4); needs ncols <= 2^32 / (256 * items_per_lane).
+     * AUTO takes it in place of SEGMENTED wherever the columns fit (COO:
+     * any algo but SEGMENTED does). */
     SP_ALGO_SEGMENTED_SORTED = 3,
     /* SEGMENTED_SORTED with the most frequent columns' x entries staged in
      * LDS (up to 112 KB; 96 KB for 8-byte accumulators): one 1024-thread

@@ -554,6 +554,8 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t e0 = cent[ci], n = cent[ci + 1] - e0;
     const uint32_t next_row = (e0 + n < P.nent) ? rip[e0 + n] : NONE;
+    // tiles stored in column order carry each entry's row-order position in the low bits
+    const uint32_t pb = P.pbits, pmask = (1u << pb) - 1;
 
     uint32_t rv[IPT], cv[IPT];
     T vv[IPT];
@@ -570,16 +572,17 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
         T xv[IPT];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < n) xv[q] = x[(size_t)cv[q] * ldx];
+            if (tid + q * TPB < n) xv[q] = x[(size_t)(cv[q] >> pb) * ldx];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < n) s_prod[tid + q * TPB] = mul<T>(vv[q], xv[q]);
+            if (tid + q * TPB < n) s_prod[pb ? (cv[q] & pmask) : tid + q * TPB] = mul<T>(vv[q], xv[q]);
     } else {
 #pragma unroll
         for (int q = 0; q < IPT; q++)
             if (tid + q * TPB < n) {
-                s_col[tid + q * TPB] = cv[q];
-                s_val[tid + q * TPB] = vv[q];
+                const uint32_t pos = pb ? (cv[q] & pmask) : tid + q * TPB;
+                s_col[pos] = cv[q] >> pb;
+                s_val[pos] = vv[q];
             }
     }
 #pragma unroll
@@ -783,10 +786,37 @@ __global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__
     for (uint32_t I = r0 + wid; I < r1; I += NW) {
         const uint32_t b = rp[I], e = rp[I + 1];
         A acc = A(0);
-        // U wave steps (about 32 blocks) per iteration with all loads issued
-        // first; lanes past the row's end reload its last block and drop the product
+        // long rows (power-law hubs): 4 wave steps per iteration, loads first,
+        // while the whole window lies in the row (the same test on every lane)
+        uint32_t base = b;
+        for (; base + 4 * CO::PB <= e; base += 4 * CO::PB) {
+            const uint32_t k = base + blk_off;
+            uint32_t J[4];
+            T v[4][CO::CH], xs[4][CO::CH];
+#pragma unroll
+            for (int u = 0; u < 4; u++) J[u] = ldg(cip + k + u * CO::PB);
+#pragma unroll
+            for (int u = 0; u < 4; u++) ldg_n<T, CO::CH>(bv + (size_t)(k + u * CO::PB) * (R * C) + g * CO::CH, v[u]);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t col = J[u] * C + c0;
+                if (ldx == 1 && col + CO::CH <= ncols) {
+                    ld_n<T, CO::CH>(x + col, xs[u]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < CO::CH; j++)
+                        xs[u][j] = col + j < ncols ? x[(size_t)(col + j) * ldx] : T(0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int j = 0; j < CO::CH; j++) acc = fma_acc<T>(v[u][j], xs[u][j], acc);
+        }
+        // the rest: U wave steps (about 32 blocks) per iteration with all loads
+        // issued first; lanes past the row's end reload its last block and drop the product
         constexpr int U = CO::PB >= 32 ? 1 : 32 / CO::PB > 8 ? 8 : 32 / CO::PB;
-        for (uint32_t k = b + blk_off; k < e; k += U * CO::PB) {
+        for (uint32_t k = base + blk_off; k < e; k += U * CO::PB) {
             uint32_t J[U];
             T v[U][CO::CH], xs[U][CO::CH];
 #pragma unroll

@@ -174,16 +174,17 @@ void tile_range(uint32_t e0, uint32_t e1, uint32_t tile, std::vector<uint32_t> &
 // col << pb | (k - j0); each tile's keys are sorted (column order, ties in row
 // order) and its values permuted to match.  Tiles are independent, so they
 // are spread over host threads.
-void column_ordered_tiles(const sp_csr *A, const std::vector<uint32_t> &cnnz, uint32_t pb, size_t sv,
-                          std::vector<uint32_t> &key, std::vector<char> &val) {
-    key.resize(A->nnz);
-    val.resize((size_t)A->nnz * sv);
+void column_ordered_tiles(const uint32_t *colind, const void *values, uint32_t nnz,
+                          const std::vector<uint32_t> &cnnz, uint32_t pb, size_t sv, std::vector<uint32_t> &key,
+                          std::vector<char> &val) {
+    key.resize(nnz);
+    val.resize((size_t)nnz * sv);
     const size_t nt = cnnz.empty() ? 0 : cnnz.size() - 1;
-    const char *src = (const char *)A->values;
+    const char *src = (const char *)values;
     auto work = [&](size_t t0, size_t step) {
         for (size_t t = t0; t < nt; t += step) {
             const uint32_t j0 = cnnz[t], j1 = cnnz[t + 1];
-            for (uint32_t k = j0; k < j1; k++) key[k] = (A->colind[k] << pb) | (k - j0);
+            for (uint32_t k = j0; k < j1; k++) key[k] = (colind[k] << pb) | (k - j0);
             std::sort(key.begin() + j0, key.begin() + j1);
             const uint32_t mask = (1u << pb) - 1;
             for (uint32_t k = j0; k < j1; k++)
@@ -350,7 +351,7 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
         if (A->nrows) merge_coords(A->rowptr, A->nrows, A->nnz, TPB * I, crow, cnnz);
         std::vector<uint32_t> key;
         std::vector<char> val;
-        column_ordered_tiles(A, cnnz, pb, sv, key, val);
+        column_ordered_tiles(A->colind, A->values, A->nnz, cnnz, pb, sv, key, val);
         if ((rc = plan_upload(p, key.data(), (size_t)A->nnz * 4, &d))) return rc;
         P.cind = (const uint32_t *)d;
         if ((rc = plan_upload(p, val.data(), (size_t)A->nnz * sv, &d))) return rc;
@@ -433,20 +434,18 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
     void *d;
     Builder b(p);
     do {
-        if ((rc = plan_upload(p, A->rowind, (size_t)nnz * 4, &d))) break;
-        P.rind = (const uint32_t *)d;
-        if ((rc = plan_upload(p, A->colind, (size_t)nnz * 4, &d))) break;
-        P.cind = (const uint32_t *)d;
-        if ((rc = plan_upload(p, A->values, (size_t)nnz * sv, &d))) break;
-        P.val = d;
         P.nrows = P.nrows_elem = n;
         P.nent = nnz;
         P.ncols_elem = A->ncols;
         const uint32_t I = pick_ipt(p->opts, 4);  // 4 items per lane: the tuner's usual winner for 1D COO
         if (!I) { rc = SP_ERR_ARG; break; }
         p->ipt = p->work.ipt = I;
-        p->algo = SP_ALGO_SEGMENTED;
         const uint32_t tile = TPB * I;
+        // Column-ordered tiles (as for CSR) unless the plain segmented order is asked for or the
+        // columns do not fit next to the position bits.
+        uint32_t pb = 0;
+        while ((1u << pb) < tile) pb++;
+        bool col_order = p->algo != SP_ALGO_SEGMENTED && A->ncols <= (1ull << (32 - pb));
         const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
         // part entry ranges
         std::vector<uint32_t> pe0(P1), pe1(P1);
@@ -489,6 +488,26 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
         if (rc) break;
         push_cut(units, nnz);
         if (nnz == 0) units.clear();
+        for (size_t u = 0; u + 1 < units.size(); u++)
+            if (units[u + 1] - units[u] > tile) col_order = false;  // positions must fit pb bits
+        p->algo = col_order ? SP_ALGO_SEGMENTED_SORTED : SP_ALGO_SEGMENTED;
+        if ((rc = plan_upload(p, A->rowind, (size_t)nnz * 4, &d))) break;
+        P.rind = (const uint32_t *)d;
+        if (col_order) {
+            std::vector<uint32_t> key;
+            std::vector<char> val;
+            column_ordered_tiles(A->colind, A->values, nnz, units, pb, sv, key, val);
+            if ((rc = plan_upload(p, key.data(), (size_t)nnz * 4, &d))) break;
+            P.cind = (const uint32_t *)d;
+            if ((rc = plan_upload(p, val.data(), (size_t)nnz * sv, &d))) break;
+            P.val = d;
+            P.pbits = pb;
+        } else {
+            if ((rc = plan_upload(p, A->colind, (size_t)nnz * 4, &d))) break;
+            P.cind = (const uint32_t *)d;
+            if ((rc = plan_upload(p, A->values, (size_t)nnz * sv, &d))) break;
+            P.val = d;
+        }
         p->nparts = P1;
         p->ntiles = units.empty() ? 0 : (uint32_t)units.size() - 1;
         for (uint32_t i = 0; i < n; i++)

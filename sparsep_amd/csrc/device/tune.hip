@@ -67,6 +67,9 @@ static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
             p.o.items_per_lane = I;
             v.push_back(p);
         }
+        Pt p{id, base};  // row-ordered tiles (the default takes column order where it fits)
+        p.o.algo = SP_ALGO_SEGMENTED;
+        v.push_back(p);
     } else if (id >= SP_K_BCSR_BLOCK && id <= SP_K_BCOO_NNZ) {
         Pt p{id, base};
         p.o.row_block_size = p.o.col_block_size = 2;

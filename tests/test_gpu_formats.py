@@ -62,15 +62,17 @@ def test_block_sizes(sp, ctx, rc, dt):
 
 @pytest.mark.parametrize("rc", [(1, 1), (2, 2), (2, 4), (4, 2), (4, 4), (8, 8)])
 @pytest.mark.parametrize("k", [16, 32, 37])
-def test_block_spmm_wide(sp, ctx, rc, k):
-    """Right-hand-side chunks of 8/16/32 with lanes across the right-hand sides
-    (R*C <= 16), and chunks of 8 in the register-blocked kernel (8x8 blocks)."""
+@pytest.mark.parametrize("dt", [3, 4, 5])
+def test_block_spmm_wide(sp, ctx, rc, k, dt):
+    """Right-hand-side chunks of 16/32: register-blocked (4-byte values, R*K <= 64),
+    lanes across the right-hand sides (8-byte values, R*C <= 16), chunks of 8 for
+    8x8 blocks.  INT64 bit-exact."""
     r, c = rc
-    A = sp.gen_stencil27(6, 2, False, sp.F32, seed=5)
-    X = np.stack([sp.gen_vector(sp.F32, A.ncols, seed=60 + q) for q in range(k)], axis=1)
+    A = sp.gen_stencil27(6, 2, False, dt, seed=5)
+    X = np.stack([sp.gen_vector(dt, A.ncols, seed=60 + q) for q in range(k)], axis=1)
     for kw in ({}, dict(lanes_per_row=1), dict(lanes_per_row=64)):
         Y, _ = run(sp, ctx, "BCSR-nnz", A, X, sp.opts(row_block_size=r, col_block_size=c, **kw), k=k)
-        assert_parity_multi(sp.F32, Y, A, X, f"{r}x{c} {kw}")
+        assert_parity_multi(dt, Y, A, X, f"{r}x{c} {kw}")
 
 
 @pytest.mark.parametrize("dt", [2, 4])
@@ -115,3 +117,24 @@ def test_unsorted_coo_input(sp, ctx):
     for kern in K1D:
         y, _ = run(sp, ctx, kern, A, x)
         assert_parity(2, y, A, x, kern)
+
+
+@pytest.mark.parametrize("dt", [2, 4, 5])
+def test_coo_column_order_same_bits(sp, ctx, dt):
+    """COO tiles in column order (the default where the columns fit) give the
+    same bits as the row-ordered tiles (algo=2), over the partition knobs and
+    for several right-hand sides."""
+    A = sp.gen_rmat(14, 150000, dtype=dt, seed=51)
+    x = sp.gen_vector(dt, A.ncols, seed=52)
+    X = np.stack([sp.gen_vector(dt, A.ncols, seed=53 + q) for q in range(3)], axis=1)
+    for kern in ("COO-row", "COO-nnz-rgrn", "COO-nnz"):
+        for oname, kw in OPTS:
+            for ipt in (4, 16):
+                y0, _ = run(sp, ctx, kern, A, x, sp.opts(algo=2, items_per_lane=ipt, **kw))
+                y1, plan = run(sp, ctx, kern, A, x, sp.opts(items_per_lane=ipt, **kw))
+                assert plan.info()["algo"] == 3
+                assert_parity(dt, y1, A, x, f"{kern}/{oname}/ipt{ipt}")
+                assert np.array_equal(y0, y1), f"{kern}/{oname}/ipt{ipt}"
+        Y0, _ = run(sp, ctx, kern, A, X, sp.opts(algo=2), k=3)
+        Y1, _ = run(sp, ctx, kern, A, X, sp.opts(), k=3)
+        assert np.array_equal(Y0, Y1)
