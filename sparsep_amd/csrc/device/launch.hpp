@@ -114,7 +114,10 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
     case WK_BCOO_SEG: {
         if (K != 1) return SP_ERR_UNSUPPORTED;
 #define SPD_BO(Rv, Cv) \
-    if (w.R == Rv && w.C == Cv) { hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 4>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break; }
+    if (w.R == Rv && w.C == Cv) { \
+        if (w.ipt == 2) hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 2>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); \
+        else hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 4>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); \
+        break; }
         do {
             SPD_BO(1, 1) SPD_BO(2, 2) SPD_BO(4, 4) SPD_BO(8, 8) SPD_BO(2, 4) SPD_BO(4, 2)
             return SP_ERR_UNSUPPORTED;

@@ -198,6 +198,11 @@ void column_ordered_tiles(const uint32_t *colind, const void *values, uint32_t n
     for (auto &t : th) t.join();
 }
 
+// Blocks per lane of the BCOO kernel: its LDS holds R sums per block of the
+// tile, so tall blocks with 8-byte sums take half the tile (FP64 4x4: 50 KB ->
+// 26 KB of LDS, 3 -> 6 workgroups per CU).
+uint32_t bcoo_ipt(int dt, uint32_t R) { return acc_size(dt) * R >= 32 ? 2 : 4; }
+
 // Staged-x slots of the hot-x kernel for a value type.
 uint32_t hot_slots(int dt) {
     switch (dt) {
@@ -615,8 +620,8 @@ int build_bcoo_plan(sp_plan *p, const sp_bcoo *B) {
     p->algo = SP_ALGO_SEGMENTED;
     p->work.R = R;
     p->work.C = C;
-    p->ipt = p->work.ipt = 4;
-    const uint32_t tile = TPB * 4;
+    p->ipt = p->work.ipt = bcoo_ipt(B->dtype, R);
+    const uint32_t tile = TPB * p->work.ipt;
     const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
     std::vector<uint32_t> bs(P1 + 1), ns(P1);
     if ((rc = sp_partition_bcoo(B->nnz_per_block, nb, B->nnz, P1, p->id == SP_K_BCOO_NNZ, bs.data(), ns.data())))
@@ -836,7 +841,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             L = coal ? 64 : avg >= 48 ? 64 : avg >= 12 ? 16 : avg >= 3 ? 4 : 1;
         }
         p->lanes = p->work.lanes = coal ? 0 : L;
-        p->ipt = p->work.ipt = kind == WK_BCOO_SEG ? 4 : I;
+        p->ipt = p->work.ipt = kind == WK_BCOO_SEG ? bcoo_ipt(A0->dtype, R) : I;
         p->work.R = R;
         p->work.C = C;
         const uint32_t tile_items = TPB * p->work.ipt;
