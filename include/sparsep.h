@@ -474,7 +474,10 @@ int sp_plan_get_info(const sp_plan *plan, sp_plan_info *info);
 
 /* y = A x on the ctx stream; x, y are device pointers of the plan dtype.
  * Replaces x push + dpu_launch + y pull + host merge
- * (spmv/1D/CSR-row/host/app.c:257-287, spmv/2D/DCSR/host/app.c:340-351). */
+ * (spmv/1D/CSR-row/host/app.c:257-287, spmv/2D/DCSR/host/app.c:340-351).
+ * A plan keeps per-call device state (carry slots, partial vectors, the
+ * staged x of hot-x plans), so calls on one plan are ordered on its
+ * context's stream; run two plans to overlap work on two streams. */
 int sp_spmv(sp_plan *plan, const void *x, void *y);
 
 /* Y = A X for k right-hand sides; X is ncols x k row-major with leading
