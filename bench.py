@@ -264,16 +264,21 @@ def wl_rmat2d(sp, ctx, args, orc):
     xd = sp.DeviceArray.from_numpy(ctx, x)
     yd = sp.DeviceArray(ctx, A.nrows, np.float32)
     for kname in ("RBDCSR", "BDCSR", "BDCOO", "DCSR"):
-        for vert in (2, 4, 8, 16, 32):
-            plan = sp.Plan(ctx, kname, A, sp.opts(vert=vert))
+        # 128 stripes of 8192 columns: the widest x slice that fits 32 KB of LDS, with and without it
+        for vert, sl in ((2, 0), (4, 0), (8, 0), (16, 0), (32, 0), (128, 0), (128, 8192)):
+            try:
+                plan = sp.Plan(ctx, kname, A, sp.opts(vert=vert, lds_slice=sl))
+            except sp._lib.SparsePError as e:
+                log(f"[bench] rmat2d {kname:8s} vert {vert:3d} slice {sl}: not built ({e})")
+                continue
             info = plan.info()
             mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
             ok = parity_ok(orc, sp.F32, A, x, yd.numpy())
             csr_bytes = 146800644
-            log(f"[bench] rmat2d {kname:8s} vert {vert:2d} {mean_ms * 1e3:8.2f} us (dominant {mean_dom * 1e3:8.2f}) "
-                f"CSR-equivalent {csr_bytes / (mean_ms * 1e-3) / 1e9:6.0f} GB/s parity {ok}")
-            cfg = {"workload": f"R-MAT 2^20 FP32, {kname} vert {vert}", "kernel": kname, "vert": vert,
-                   "algo_bytes": info["algo_bytes"]}
+            log(f"[bench] rmat2d {kname:8s} vert {vert:3d} slice {sl:4d} {mean_ms * 1e3:8.2f} us (dominant "
+                f"{mean_dom * 1e3:8.2f}) CSR-equivalent {csr_bytes / (mean_ms * 1e-3) / 1e9:6.0f} GB/s parity {ok}")
+            cfg = {"workload": f"R-MAT 2^20 FP32, {kname} vert {vert}" + (f", x slices in LDS" if sl else ""),
+                   "kernel": kname, "vert": vert, "lds_slice": sl, "algo_bytes": info["algo_bytes"]}
             out.append(line(args, info["algo_bytes"] / (mean_ms * 1e-3) / 1e9, mean_ms, "f32", cfg,
                             extra={"parity_vs_oracle": ok, "launches": info["launches"],
                                    "csr_equivalent_gbs": round(csr_bytes / (mean_ms * 1e-3) / 1e9, 1)}))

@@ -423,7 +423,10 @@ typedef struct sp_opts {
     uint32_t vert;            /* 2D vertical partitions (-v); 0 = auto          */
     uint32_t row_block_size;  /* B*: block rows (-r), default 4                 */
     uint32_t col_block_size;  /* B*: block cols (-c), default 4                 */
-    uint32_t lds_slice;       /* 2D: max x elements staged in LDS; 0 = auto     */
+    uint32_t lds_slice;       /* 2D CSR/COO: stage each tile's x slice in LDS   *
+                               * when every stripe has at most this many        *
+                               * columns (and 32 KB); wider: SP_ERR_UNSUPPORTED. *
+                               * 0 = off (gathers from global memory)           */
     int32_t count_values;     /* B*: nnz_per_block counts non-zero values       */
     uint64_t mem_cap_bytes;   /* fail with SP_ERR_NOMEM above this; 0 = none    */
 } sp_opts;

@@ -136,10 +136,17 @@ __device__ __host__ inline uint32_t merge_search(uint32_t d, uint32_t na, uint32
     return lo;
 }
 
+// LDS bytes for a staged x slice (2D tiles): three workgroups per CU fit
+// next to the tile buffers.
+constexpr int SLICE_BYTES = 32768;
+
 // ---------------------------------------------------------------------------
-// CSR, entry balanced (merge path)
+// CSR, entry balanced (merge path).  SLICE (2D tiles, one right-hand side):
+// each workgroup takes a contiguous run of units, copies the x slice of the
+// current tile (its ncols_elem columns) into LDS when the tile changes, and
+// gathers from LDS; the sums are the same as without the slice.
 // ---------------------------------------------------------------------------
-template <typename T, int IPT, int K>
+template <typename T, int IPT, int K, bool SLICE = false>
 __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__ probs,
                                                         const uint32_t *__restrict__ unit_pid,
                                                         const uint32_t *__restrict__ crow,
@@ -159,11 +166,21 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     __shared__ uint32_t s_key[TPB];
     __shared__ A s_scan[K][TPB];
     __shared__ A s_wc[K][TPB / WAVE];
+    __shared__ T s_x[SLICE ? SLICE_BYTES / (int)sizeof(T) : 1];
 
     const int tid = threadIdx.x;
     // Grid-stride over tiles: one tile per workgroup for a plain launch, a
-    // persistent sweep when the grid is capped (batched launches).
-    for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
+    // persistent sweep when the grid is capped (batched launches); with a
+    // slice, contiguous runs so that consecutive units share the slice.
+    uint32_t t0 = blockIdx.x, t1 = nunits, tstep = gridDim.x;
+    if constexpr (SLICE) {
+        const uint32_t per = (nunits + gridDim.x - 1) / gridDim.x;
+        t0 = min(nunits, blockIdx.x * per);
+        t1 = min(nunits, t0 + per);
+        tstep = 1;
+    }
+    [[maybe_unused]] uint32_t staged = NONE;
+    for (uint32_t t = t0; t < t1; t += tstep) {
     const uint32_t pid = unit_pid[t], ci = t + pid;
     const Prob P = probs[pid];
     const gptr<const T> __restrict__ x = gp(io.x(pid, P));
@@ -171,6 +188,13 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     [[maybe_unused]] const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
     const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
+    if constexpr (SLICE) {
+        if (pid != staged) {  // the previous unit ended with a barrier, so s_x is free
+            for (uint32_t c = tid; c < P.ncols_elem; c += TPB) s_x[c] = x[(size_t)c * ldx];
+            __syncthreads();
+            staged = pid;
+        }
+    }
     const uint32_t i0 = crow[ci], j0 = cnnz[ci];
     const uint32_t na = crow[ci + 1] - i0, nb = cnnz[ci + 1] - j0;
     // tiles stored in column order carry each entry's row-order position in the low bits
@@ -198,7 +222,7 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
         T xv[IPT];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < nb) xv[q] = x[(size_t)(cv[q] >> pb) * ldx];
+            if (tid + q * TPB < nb) xv[q] = SLICE ? s_x[cv[q] >> pb] : x[(size_t)(cv[q] >> pb) * ldx];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
             if (tid + q * TPB < nb) s_prod[pb ? (cv[q] & pmask) : tid + q * TPB] = mul<T>(vv[q], xv[q]);
@@ -526,7 +550,7 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
 // thread holding its last entry, with the partial sums of earlier threads
 // of the tile added in; partials of earlier tiles arrive in the fix-up.
 // ---------------------------------------------------------------------------
-template <typename T, int IPT, int K>
+template <typename T, int IPT, int K, bool SLICE = false>
 __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ probs,
                                                       const uint32_t *__restrict__ unit_pid,
                                                       const uint32_t *__restrict__ cent, Ptrs<T> io,
@@ -542,9 +566,19 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     __shared__ uint32_t s_key[TPB];
     __shared__ A s_scan[K][TPB];
     __shared__ A s_wc[K][TPB / WAVE];
+    __shared__ T s_x[SLICE ? SLICE_BYTES / (int)sizeof(T) : 1];
 
     const int tid = threadIdx.x;
-    for (uint32_t t = blockIdx.x; t < nunits; t += gridDim.x) {
+    // as csr_merge_kernel: grid-stride, or contiguous runs sharing a staged x slice
+    uint32_t t0 = blockIdx.x, t1 = nunits, tstep = gridDim.x;
+    if constexpr (SLICE) {
+        const uint32_t per = (nunits + gridDim.x - 1) / gridDim.x;
+        t0 = min(nunits, blockIdx.x * per);
+        t1 = min(nunits, t0 + per);
+        tstep = 1;
+    }
+    [[maybe_unused]] uint32_t staged = NONE;
+    for (uint32_t t = t0; t < t1; t += tstep) {
     const uint32_t pid = unit_pid[t], ci = t + pid;
     const Prob P = probs[pid];
     const gptr<const T> __restrict__ x = gp(io.x(pid, P));
@@ -552,6 +586,13 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     [[maybe_unused]] const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
     const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
+    if constexpr (SLICE) {
+        if (pid != staged) {  // the previous unit ended with a barrier, so s_x is free
+            for (uint32_t c = tid; c < P.ncols_elem; c += TPB) s_x[c] = x[(size_t)c * ldx];
+            __syncthreads();
+            staged = pid;
+        }
+    }
     const uint32_t e0 = cent[ci], n = cent[ci + 1] - e0;
     const uint32_t next_row = (e0 + n < P.nent) ? rip[e0 + n] : NONE;
     // tiles stored in column order carry each entry's row-order position in the low bits
@@ -572,7 +613,7 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
         T xv[IPT];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < n) xv[q] = x[(size_t)(cv[q] >> pb) * ldx];
+            if (tid + q * TPB < n) xv[q] = SLICE ? s_x[cv[q] >> pb] : x[(size_t)(cv[q] >> pb) * ldx];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
             if (tid + q * TPB < n) s_prod[pb ? (cv[q] & pmask) : tid + q * TPB] = mul<T>(vv[q], xv[q]);

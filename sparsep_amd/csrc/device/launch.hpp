@@ -63,6 +63,16 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
     }
     switch (w.kind) {
     case WK_CSR_MERGE:
+        if constexpr (K == 1) {
+            if (w.slice) {
+                switch (w.ipt) {
+                case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, 1, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+                case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, 1, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+                default: return SP_ERR_ARG;
+                }
+                break;
+            }
+        }
         switch (w.ipt) {
         case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
         case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
@@ -79,6 +89,16 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
 #undef SPD_VEC
         break;
     case WK_COO_SEG:
+        if constexpr (K == 1) {
+            if (w.slice) {
+                switch (w.ipt) {
+                case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, 1, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+                case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, 1, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+                default: return SP_ERR_ARG;
+                }
+                break;
+            }
+        }
         switch (w.ipt) {
         case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
         case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;

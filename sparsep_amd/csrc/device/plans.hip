@@ -832,6 +832,24 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
         }
         const uint32_t I = pick_ipt(p->opts, 8);
         if (!I) { rc = SP_ERR_ARG; break; }
+        // x slices staged in LDS (2D CSR/COO, the reference's per-DPU x slice,
+        // spmv/2D/DCSR/host/app.c:308-315): every stripe must fit the asked slice
+        // and the kernels' LDS budget; the merge-path kernel carries the CSR case
+        if (p->opts.lds_slice && (tk.format == SP_FMT_CSR || tk.format == SP_FMT_COO)) {
+            uint32_t maxw = 0;
+            for (uint32_t c = 0; c < vert; c++)
+                maxw = std::max(maxw, T->vert_tile_widths[c + 1] - T->vert_tile_widths[c]);
+            if (maxw > p->opts.lds_slice || maxw > spd::SLICE_BYTES / sv || I > 8) {
+                rc = SP_ERR_UNSUPPORTED;
+                break;
+            }
+            if (kind == WK_CSR_VEC) {
+                kind = WK_CSR_MERGE;
+                p->algo = SP_ALGO_SEGMENTED;
+            }
+            p->work.slice = true;
+            p->work.grid_cap = 3 * (uint32_t)std::max(1, p->ctx->num_cus);  // three workgroups per CU fit
+        }
         uint32_t L = p->opts.lanes_per_row;
         if (kind == WK_CSR_VEC && L == 0) L = auto_lanes(n ? (double)A0->nnz / n / vert : 1.0);
         bool coal = false;

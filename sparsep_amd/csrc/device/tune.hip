@@ -80,6 +80,19 @@ static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
             p.o.vert = vert;
             v.push_back(p);
         }
+        // 2D CSR/COO: x slices in LDS, with the fewest stripes (up to 64) whose
+        // even width fits 32 KB of values
+        const bool csr_coo = id == SP_K_DCSR || id == SP_K_DCOO || id == SP_K_RBDCSR || id == SP_K_RBDCOO ||
+                             id == SP_K_BDCSR || id == SP_K_BDCOO;
+        const uint32_t cap = 32768u / (uint32_t)sp_dtype_size(A->dtype);
+        uint32_t vs = 2;
+        while (vs < 64 && ((uint64_t)A->ncols + vs - 1) / vs > cap) vs *= 2;
+        if (csr_coo && ((uint64_t)A->ncols + vs - 1) / vs <= cap) {
+            Pt p{id, base};
+            p.o.vert = vs;
+            p.o.lds_slice = cap;
+            v.push_back(p);
+        }
     }
 }
 

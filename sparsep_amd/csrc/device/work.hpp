@@ -31,6 +31,7 @@ struct Work {
     bool zero_all = false;  // zero every output row first (2D partial buffers)
     uint64_t zero_rows = 0; // rows to zero when zero_all (times ldy elements)
     uint32_t grid_cap = 0;  // persistent launch: at most this many workgroups (0 = one per unit)
+    bool slice = false;     // 2D CSR/COO tiles: one right-hand side gathers from an LDS copy of the tile's x slice
 };
 
 // Widest right-hand-side chunk a kernel kind takes in one pass: the

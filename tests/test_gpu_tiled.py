@@ -62,3 +62,32 @@ def test_2d_edge_cases(sp, ctx):
             for kw in (dict(vert=4), dict(vert=16, nparts=64)):
                 y, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
                 assert_parity(2, y, A, x, f"{nr}x{nc}/{kern}/{kw}")
+
+
+@pytest.mark.parametrize("dt", [0, 2, 4, 5])
+def test_lds_x_slices(sp, ctx, dt):
+    """x slices staged in LDS (lds_slice) for the 2D CSR and COO tilings give
+    the same bits as gathers from global memory, with one and several
+    right-hand sides (several run without the slice), and a stripe wider than
+    the slice is refused."""
+    A = sp.gen_rmat(14, 200000, dtype=dt, seed=61)       # 16384 columns
+    x = sp.gen_vector(dt, A.ncols, seed=62)
+    X = np.stack([sp.gen_vector(dt, A.ncols, seed=63 + q) for q in range(3)], axis=1)
+    cap = 32768 // np.dtype(sp.NP_DTYPE[dt]).itemsize
+    vert = max(2, -(-A.ncols // cap))
+    vert = 1 << (vert - 1).bit_length()
+    for kern in ("DCSR", "DCOO", "RBDCSR", "RBDCOO", "BDCSR", "BDCOO"):
+        for kw in (dict(), dict(items_per_lane=4), dict(nparts=64, nsub=4)):
+            base = dict(vert=2 * vert, **kw)   # even stripes at most cap / 2 wide; BD stripes may vary
+            y0, _ = run(sp, ctx, kern, A, x, sp.opts(**base))
+            try:
+                y1, _ = run(sp, ctx, kern, A, x, sp.opts(lds_slice=cap, **base))
+            except sp._lib.SparsePError as e:
+                assert kern.startswith("BD") and e.rc == -5, (kern, kw)  # a variable stripe too wide
+                continue
+            assert_parity(dt, y1, A, x, f"{kern}/{kw}/slice")
+            assert np.array_equal(y0, y1), f"{kern}/{kw}"
+            Y1, _ = run(sp, ctx, kern, A, X, sp.opts(lds_slice=cap, **base), k=3)
+            assert_parity_multi(dt, Y1, A, X, f"{kern}/{kw}/slice spmm")
+    with pytest.raises(sp._lib.SparsePError):
+        sp.Plan(ctx, "RBDCSR", A, sp.opts(vert=2, lds_slice=A.ncols // 4))
