@@ -189,9 +189,10 @@ extern "C" int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, co
         if (!a.c1.empty() && (rc = bupload(b, a.c1, &w.d_c1))) break;
         if (!a.c2.empty() && (rc = bupload(b, a.c2, &w.d_c2))) break;
         if (w.nzero && ((rc = bupload(b, a.zpid, &w.d_zpid)) || (rc = bupload(b, a.zrow, &w.d_zrow)))) break;
-        const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG || w.kind == WK_CSR_HOT;
+        const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG || w.kind == WK_CSR_HOT ||
+                             w.kind == WK_BCSR_SEG;
         if (carries) {
-            const uint32_t S = w.kind == WK_BCOO_SEG ? w.R : 1;
+            const uint32_t S = (w.kind == WK_BCOO_SEG || w.kind == WK_BCSR_SEG) ? w.R : 1;
             const size_t n = (size_t)std::max<uint32_t>(w.nunits, 1) * S;
             const size_t asz = (dt == SP_I64 || dt == SP_F64) ? 8 : 4;
             w.carry_k = (uint32_t)g.kc;

@@ -973,18 +973,27 @@ __global__ __launch_bounds__(TPB) void rows_rhs_kernel(const Prob *__restrict__ 
 // tile with lane groups of C over the block's elements (coalesced value
 // loads); phase 2 is the COO segmented walk over blocks with R-wide sums.
 // ---------------------------------------------------------------------------
-template <typename T, int R, int C, int IPT>
+//
+// PTR (BCSR, entry balanced): the same kernel over merge-path tiles of a BCSR
+// matrix (block-row ends + blocks, coordinates crow = cent and cnnz); a
+// block's row comes from a binary search over the tile's block-row ends in
+// LDS instead of a stored row index, and a tile's last row is complete when
+// its end lies in the tile.  Block rows without blocks are zeroed by the
+// fix-up.
+template <typename T, int R, int C, int IPT, bool PTR = false>
 __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ probs,
                                                        const uint32_t *__restrict__ unit_pid,
-                                                       const uint32_t *__restrict__ cent, Ptrs<T> io,
+                                                       const uint32_t *__restrict__ cent,
+                                                       const uint32_t *__restrict__ cnnz, Ptrs<T> io,
                                                        uint32_t *__restrict__ carry_row,
                                                        acc_t<T> *__restrict__ carry_val,
                                                        uint32_t nunits) {
     using A = acc_t<T>;
-    constexpr int TILE = TPB * IPT;  // blocks per tile
+    constexpr int TILE = TPB * IPT;  // blocks (BCOO) or block-row ends + blocks (PTR) per tile
     static_assert(TPB % (R * C) == 0, "block must tile the workgroup");
     __shared__ uint32_t s_brow[TILE];
     __shared__ uint32_t s_bcol[TILE];
+    __shared__ uint32_t s_rowend[PTR ? TILE + 1 : 1];
     __shared__ A s_part[TILE * R];
     __shared__ uint32_t s_key[TPB];
     __shared__ A s_scan[R][TPB];
@@ -999,15 +1008,46 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
     [[maybe_unused]] const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), rip = gp(P.rind), cip = gp(P.cind);
     const gptr<const T> __restrict__ bv = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
-    const uint32_t b0 = cent[ci], n = cent[ci + 1] - b0;
-    const uint32_t next_brow = (b0 + n < P.nent) ? rip[b0 + n] : NONE;
-
+    uint32_t b0, n, next_brow, i0 = 0, na = 0;
+    if constexpr (PTR) {
+        i0 = cent[ci];
+        na = cent[ci + 1] - i0;
+        b0 = cnnz[ci];
+        n = cnnz[ci + 1] - b0;
+        next_brow = NONE;  // unused: completeness comes from the row ends
+        const uint32_t nrl = min(na + 1, P.nrows - i0);  // the ends of rows i0 .. i0 + na
 #pragma unroll
-    for (int q = 0; q < IPT; q++) {
-        const uint32_t k = tid + q * TPB;
-        if (k < n) {
-            s_brow[k] = ldg(rip + b0 + k);
-            s_bcol[k] = ldg(cip + b0 + k);
+        for (int q = 0; q <= IPT; q++) {
+            const uint32_t k = tid + q * TPB;
+            if (k < nrl) s_rowend[k] = rp[i0 + k + 1];
+            if (q < IPT && k < n) s_bcol[k] = ldg(cip + b0 + k);
+        }
+        __syncthreads();
+        // row of block b0 + k: i0 + the number of the tile's row ends at or before it
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const uint32_t k = tid + q * TPB;
+            if (k < n) {
+                uint32_t lo = 0, hi = na;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_rowend[mid] <= b0 + k) lo = mid + 1;
+                    else hi = mid;
+                }
+                s_brow[k] = i0 + lo;
+            }
+        }
+    } else {
+        b0 = cent[ci];
+        n = cent[ci + 1] - b0;
+        next_brow = (b0 + n < P.nent) ? rip[b0 + n] : NONE;
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const uint32_t k = tid + q * TPB;
+            if (k < n) {
+                s_brow[k] = ldg(rip + b0 + k);
+                s_bcol[k] = ldg(cip + b0 + k);
+            }
         }
     }
     __syncthreads();
@@ -1105,8 +1145,10 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
         for (int i = 0; i < R; i++) acc[i] = acc[i] + s_part[e * R + i];
     }
     if (a < bb) {
-        const uint32_t nxt = bb < n ? s_brow[bb] : next_brow;
-        if (nxt != cur) {
+        // the thread's last row ends here if the next block is in another row (BCOO), or
+        // (last block of a PTR tile) if the row's end lies in the tile
+        const bool ends = bb < n ? s_brow[bb] != cur : PTR ? s_rowend[cur - i0] <= b0 + n : next_brow != cur;
+        if (ends) {
             final_emitted = true;
             if (!emitted) {
                 emitted = true;

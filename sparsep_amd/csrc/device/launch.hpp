@@ -131,12 +131,26 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
 #undef SPD_BL
         break;
     }
+    case WK_BCSR_SEG: {
+        if (K != 1) return SP_ERR_UNSUPPORTED;
+#define SPD_BS(Rv, Cv) \
+    if (w.R == Rv && w.C == Cv) { \
+        if (w.ipt == 2) hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 2, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); \
+        else hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 4, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); \
+        break; }
+        do {
+            SPD_BS(1, 1) SPD_BS(2, 2) SPD_BS(4, 4) SPD_BS(8, 8) SPD_BS(2, 4) SPD_BS(4, 2)
+            return SP_ERR_UNSUPPORTED;
+        } while (0);
+#undef SPD_BS
+        break;
+    }
     case WK_BCOO_SEG: {
         if (K != 1) return SP_ERR_UNSUPPORTED;
 #define SPD_BO(Rv, Cv) \
     if (w.R == Rv && w.C == Cv) { \
-        if (w.ipt == 2) hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 2>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); \
-        else hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 4>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); \
+        if (w.ipt == 2) hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 2>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, nullptr, io, w.d_carry_row, cv, w.nunits); \
+        else hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 4>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, nullptr, io, w.d_carry_row, cv, w.nunits); \
         break; }
         do {
             SPD_BO(1, 1) SPD_BO(2, 2) SPD_BO(4, 4) SPD_BO(8, 8) SPD_BO(2, 4) SPD_BO(4, 2)
@@ -160,8 +174,9 @@ static int launch_chunk(const Work &w, sp_ctx *ctx, Ptrs<T> io) {
         if (rc) return rc;
     }
     mark_dominant(ctx);
-    const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG || w.kind == WK_CSR_HOT;
-    const uint32_t S = w.kind == WK_BCOO_SEG ? w.R : 1;
+    const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG || w.kind == WK_CSR_HOT ||
+                         w.kind == WK_BCSR_SEG;
+    const uint32_t S = (w.kind == WK_BCOO_SEG || w.kind == WK_BCSR_SEG) ? w.R : 1;
     const uint64_t nthreads = (carries ? (uint64_t)S * w.nunits : 0) + w.nzero;
     if (nthreads) {
         using A = acc_t<T>;

@@ -64,7 +64,7 @@ struct Builder {
             if ((rc = plan_upload(p, zrow.data(), zrow.size() * 4, (void **)&w.d_zrow))) return rc;
         }
         if (carries) {
-            w.carry_k = kind == WK_BCOO_SEG ? 1 : 8;
+            w.carry_k = (kind == WK_BCOO_SEG || kind == WK_BCSR_SEG) ? 1 : 8;
             const size_t n = (size_t)std::max<uint32_t>(w.nunits, 1) * slots;
             if ((rc = plan_alloc(p, n * 4, (void **)&w.d_carry_row))) return rc;
             if ((rc = plan_alloc(p, n * w.carry_k * acc_size(p->dtype), &w.d_carry_val))) return rc;
@@ -544,6 +544,36 @@ int build_bcsr_plan(sp_plan *p, const sp_bcsr *B) {
     P.nent = (uint32_t)B->num_blocks;
     P.nrows_elem = B->nrows;
     P.ncols_elem = B->ncols;
+    // Entry balanced (merge-path tiles over block-row ends and blocks) when asked, or,
+    // automatically, when block rows are skewed (as for CSR): a block row per wave or
+    // lane group would leave the hub rows to a few waves.
+    uint32_t maxlen = 0;
+    for (uint32_t I = 0; I < nbr; I++) maxlen = std::max(maxlen, B->browptr[I + 1] - B->browptr[I]);
+    const double avg_b = nbr ? (double)B->num_blocks / nbr : 0.0;
+    const bool seg = B->num_blocks <= 0xFFFFFFFFULL &&
+                     (p->algo == SP_ALGO_SEGMENTED ||
+                      (p->algo == SP_ALGO_AUTO && p->opts.lanes_per_row == 0 && maxlen > 4 * avg_b + 64));
+    if (seg) {
+        p->algo = SP_ALGO_SEGMENTED;
+        const uint32_t I = bcoo_ipt(B->dtype, R);
+        p->ipt = p->work.ipt = I;
+        p->work.R = R;
+        p->work.C = C;
+        std::vector<uint32_t> crow, cnnz;
+        if (nbr) merge_coords(B->browptr, nbr, (uint32_t)B->num_blocks, TPB * I, crow, cnnz);
+        p->nparts = p->ntiles = crow.empty() ? 0 : (uint32_t)crow.size() - 1;
+        p->nsub = TPB;
+        Builder b(p);
+        // element rows of block rows without blocks are zeroed by the fix-up
+        for (uint32_t I2 = 0; I2 < nbr; I2++)
+            if (B->browptr[I2 + 1] == B->browptr[I2])
+                for (uint32_t i = 0; i < R && (uint64_t)I2 * R + i < B->nrows; i++) {
+                    b.zpid.push_back(0);
+                    b.zrow.push_back(I2 * R + i);
+                }
+        b.add(P, crow, &cnnz);
+        return b.finish(WK_BCSR_SEG, true, R);
+    }
     p->algo = SP_ALGO_VECTOR;
     uint32_t L = p->opts.lanes_per_row;
     const double avg = nbr ? (double)B->num_blocks / nbr : 1.0;

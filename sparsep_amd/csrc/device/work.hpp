@@ -11,7 +11,8 @@
 
 struct sp_ctx;
 
-enum WorkKind { WK_CSR_MERGE = 0, WK_CSR_VEC = 1, WK_COO_SEG = 2, WK_BCSR_VEC = 3, WK_BCOO_SEG = 4, WK_CSR_HOT = 5 };
+enum WorkKind { WK_CSR_MERGE = 0, WK_CSR_VEC = 1, WK_COO_SEG = 2, WK_BCSR_VEC = 3, WK_BCOO_SEG = 4, WK_CSR_HOT = 5,
+               WK_BCSR_SEG = 6 };
 
 struct Work {
     int kind = WK_CSR_MERGE;
@@ -45,6 +46,7 @@ inline int max_rhs_chunk(const Work &w) {
     case WK_CSR_VEC: return 32;
     case WK_BCSR_VEC: return wide8 ? 32 : w.R <= 2 ? 32 : w.R <= 4 ? 16 : 8;
     case WK_BCOO_SEG: return 1;
+    case WK_BCSR_SEG: return 1;
     case WK_CSR_HOT: return 1;
     default: return 8;
     }

@@ -138,3 +138,36 @@ def test_coo_column_order_same_bits(sp, ctx, dt):
         Y0, _ = run(sp, ctx, kern, A, X, sp.opts(algo=2), k=3)
         Y1, _ = run(sp, ctx, kern, A, X, sp.opts(), k=3)
         assert np.array_equal(Y0, Y1)
+
+
+def _coo_rows(sp, nrows, ncols, rows, cols, dt, seed=3):
+    rng = np.random.default_rng(seed)
+    vals = rng.integers(-4, 5, size=len(rows)).astype(sp.NP_DTYPE[dt])
+    return sp.Coo.from_arrays(nrows, ncols, np.asarray(rows), np.asarray(cols), vals, sort=True)
+
+
+@pytest.mark.parametrize("rc", [(1, 1), (2, 2), (4, 4), (8, 8), (2, 4), (4, 2)])
+@pytest.mark.parametrize("dt", [0, 2, 3, 4, 5])
+def test_bcsr_segmented(sp, ctx, rc, dt):
+    """Entry-balanced BCSR (merge-path tiles over block-row ends and blocks,
+    algo=2, and the automatic choice for skewed block rows): power-law and
+    uniform matrices, a hub row spanning many tiles between empty block rows,
+    trailing empty rows, and several right-hand sides."""
+    r, c = rc
+    n = 9000
+    hub = _coo_rows(sp, 4000, n, [37] * n + [3999] * 5, list(range(n)) + [1, 2, 3, 4, 5], dt)
+    mats = [sp.gen_rmat(12, 60000, dtype=dt, seed=71), sp.gen_uniform(2001, 1999, 0, 30, dtype=dt, seed=72), hub,
+            _coo_rows(sp, 300, 50, [], [], dt)]
+    for i, A in enumerate(mats):
+        x = sp.gen_vector(dt, A.ncols, seed=73 + i)
+        for kern in ("BCSR-block", "BCSR-nnz"):
+            y, plan = run(sp, ctx, kern, A, x, sp.opts(algo=2, row_block_size=r, col_block_size=c))
+            assert plan.info()["algo"] == 2
+            assert_parity(dt, y, A, x, f"mat{i}/{kern}/{r}x{c}")
+        if i == 0:
+            X = np.stack([sp.gen_vector(dt, A.ncols, seed=80 + q) for q in range(3)], axis=1)
+            Y, _ = run(sp, ctx, "BCSR-nnz", A, X, sp.opts(algo=2, row_block_size=r, col_block_size=c), k=3)
+            assert_parity_multi(dt, Y, A, X, f"spmm {r}x{c}")
+    # the hub matrix is skewed: the default picks the entry-balanced kernel
+    _, plan = run(sp, ctx, "BCSR-nnz", hub, sp.gen_vector(dt, hub.ncols, seed=9), sp.opts(row_block_size=r, col_block_size=c))
+    assert plan.info()["algo"] == 2
