@@ -74,6 +74,11 @@ static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
         Pt p{id, base};
         p.o.row_block_size = p.o.col_block_size = 2;
         v.push_back(p);
+        if (id == SP_K_BCSR_BLOCK || id == SP_K_BCSR_NNZ) {  // entry-balanced BCSR (automatic only when skewed)
+            Pt q{id, base};
+            q.o.algo = SP_ALGO_SEGMENTED;
+            v.push_back(q);
+        }
     } else {
         for (uint32_t vert : {4u, 8u}) {
             Pt p{id, base};
