@@ -395,7 +395,10 @@ const char *sp_kernel_name(int id);
 typedef enum sp_algo {
     SP_ALGO_AUTO = 0,
     SP_ALGO_VECTOR = 1,    /* row granular: `lanes_per_row` lanes per row   */
-    SP_ALGO_SEGMENTED = 2, /* entry balanced: fixed entry tiles, rows split */
+    /* entry balanced: fixed entry tiles, rows split.  BCSR ids 5 and 6:
+     * merge-path tiles over block-row ends and blocks (AUTO takes them when
+     * the longest block row exceeds 4 * average + 64 blocks). */
+    SP_ALGO_SEGMENTED = 2,
     /* SEGMENTED with every tile's entries stored in column order, so the
      * lanes of one gather instruction share cache lines; products return to
      * row order in LDS, giving the same sums in the same order.  1D CSR

@@ -80,3 +80,27 @@ def test_batch_hot_plans(sp, ctx, dt):
         ctx.sync()
         for i, (A, k) in enumerate(probs):
             assert_parity_multi(dt, Yd[i].numpy(), A, Xs[i], f"rep{rep} prob{i}")
+
+
+@pytest.mark.parametrize("dt", [2, 4, 5])
+def test_batch_block_plans(sp, ctx, dt):
+    """Block formats in a batch: entry-balanced BCSR, row-granular BCSR and
+    BCOO plans (4x4 and 2x2 blocks) over several right-hand sides."""
+    probs = problems(sp, dt, n=6, seed=13)
+    opts = [dict(algo=2), dict(), dict(algo=2, row_block_size=2, col_block_size=2)]
+    kinds = ["BCSR-nnz", "BCSR-block", "BCOO-nnz"]
+    plans = [sp.Plan(ctx, kinds[i % 3], A, sp.opts(**opts[(i // 3) % 3])) for i, (A, k) in enumerate(probs)]
+    b = sp.Batch(ctx, plans, [k for _, k in probs], graph=True)
+    for rep in range(2):
+        Xs, Xd, Yd = [], [], []
+        for i, (A, k) in enumerate(probs):
+            X = np.stack([sp.gen_vector(dt, A.ncols, seed=500 * rep + 13 * i + q) for q in range(k)], axis=1)
+            Xs.append(X)
+            Xd.append(sp.DeviceArray.from_numpy(ctx, X))
+            y = sp.DeviceArray(ctx, (A.nrows, k), X.dtype)
+            y.fill_bytes(0x5A)
+            Yd.append(y)
+        b.run([x.addr for x in Xd], [y.addr for y in Yd])
+        ctx.sync()
+        for i, (A, k) in enumerate(probs):
+            assert_parity_multi(dt, Yd[i].numpy(), A, Xs[i], f"rep{rep} prob{i}")
