@@ -146,15 +146,15 @@ constexpr int SLICE_BYTES = 32768;
 // current tile (its ncols_elem columns) into LDS when the tile changes, and
 // gathers from LDS; the sums are the same as without the slice.
 // ---------------------------------------------------------------------------
-template <typename T, int IPT, int K, bool SLICE = false>
-__global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__ probs,
+template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB>
+__global__ __launch_bounds__(NT) void csr_merge_kernel(const Prob *__restrict__ probs,
                                                         const uint32_t *__restrict__ unit_pid,
                                                         const uint32_t *__restrict__ crow,
                                                         const uint32_t *__restrict__ cnnz, Ptrs<T> io,
                                                         uint32_t *__restrict__ carry_row,
                                                         acc_t<T> *__restrict__ carry_val, uint32_t nunits) {
     using A = acc_t<T>;
-    constexpr int TILE = TPB * IPT;
+    constexpr int TILE = NT * IPT;
     // One right-hand side: stage products (the x gather happens in the
     // coalesced load phase).  Several: stage (column, value) pairs and
     // gather x rows during the walk.
@@ -163,9 +163,9 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     __shared__ A s_prod[PROD ? TILE : 1];
     __shared__ uint32_t s_col[PROD ? 1 : TILE];
     __shared__ T s_val[PROD ? 1 : TILE];
-    __shared__ uint32_t s_key[TPB];
-    __shared__ A s_scan[K][TPB];
-    __shared__ A s_wc[K][TPB / WAVE];
+    __shared__ uint32_t s_key[NT];
+    __shared__ A s_scan[K][NT];
+    __shared__ A s_wc[K][NT / WAVE];
     __shared__ T s_x[SLICE ? SLICE_BYTES / (int)sizeof(T) : 1];
 
     const int tid = threadIdx.x;
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     const int ldx = io.lx(P), ldy = io.ly(P);
     if constexpr (SLICE) {
         if (pid != staged) {  // the previous unit ended with a barrier, so s_x is free
-            for (uint32_t c = tid; c < P.ncols_elem; c += TPB) s_x[c] = x[(size_t)c * ldx];
+            for (uint32_t c = tid; c < P.ncols_elem; c += NT) s_x[c] = x[(size_t)c * ldx];
             __syncthreads();
             staged = pid;
         }
@@ -207,12 +207,12 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
     T vv[IPT];
 #pragma unroll
     for (int q = 0; q <= IPT; q++) {
-        const uint32_t k = tid + q * TPB;
+        const uint32_t k = tid + q * NT;
         if (k < nrl) re[q] = rp[i0 + k + 1];
     }
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
-        const uint32_t k = tid + q * TPB;
+        const uint32_t k = tid + q * NT;
         if (k < nb) {
             cv[q] = ldg(cip + j0 + k);
             vv[q] = ldg(vals + j0 + k);
@@ -222,22 +222,22 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
         T xv[IPT];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < nb) xv[q] = SLICE ? s_x[cv[q] >> pb] : x[(size_t)(cv[q] >> pb) * ldx];
+            if (tid + q * NT < nb) xv[q] = SLICE ? s_x[cv[q] >> pb] : x[(size_t)(cv[q] >> pb) * ldx];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < nb) s_prod[pb ? (cv[q] & pmask) : tid + q * TPB] = mul<T>(vv[q], xv[q]);
+            if (tid + q * NT < nb) s_prod[pb ? (cv[q] & pmask) : tid + q * NT] = mul<T>(vv[q], xv[q]);
     } else {
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < nb) {
-                const uint32_t pos = pb ? (cv[q] & pmask) : tid + q * TPB;
+            if (tid + q * NT < nb) {
+                const uint32_t pos = pb ? (cv[q] & pmask) : tid + q * NT;
                 s_col[pos] = cv[q] >> pb;
                 s_val[pos] = vv[q];
             }
     }
 #pragma unroll
     for (int q = 0; q <= IPT; q++)
-        if (tid + q * TPB < nrl) s_rowend[tid + q * TPB] = re[q];
+        if (tid + q * NT < nrl) s_rowend[tid + q * NT] = re[q];
     __syncthreads();
 
     const uint32_t total = na + nb;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
         }
     }
     const uint32_t key = i0 + li;
-    block_seg_scan<A, K>(key, acc, s_key, s_scan, s_wc);
+    block_seg_scan<A, K, NT>(key, acc, s_key, s_scan, s_wc);
     if (emitted) {
 #pragma unroll
         for (int q = 0; q < K; q++) {
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(TPB) void csr_merge_kernel(const Prob *__restrict__
             y[(size_t)first_row * ldy + q] = to_val<T>(cin + first[q]);
         }
     }
-    if (tid == TPB - 1) {
+    if (tid == NT - 1) {
         carry_row[t] = key < P.nrows ? key : NONE;
 #pragma unroll
         for (int q = 0; q < K; q++) carry_val[(size_t)t * K + q] = acc[q];
