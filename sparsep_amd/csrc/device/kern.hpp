@@ -550,22 +550,22 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
 // thread holding its last entry, with the partial sums of earlier threads
 // of the tile added in; partials of earlier tiles arrive in the fix-up.
 // ---------------------------------------------------------------------------
-template <typename T, int IPT, int K, bool SLICE = false>
-__global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ probs,
+template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB>
+__global__ __launch_bounds__(NT) void coo_seg_kernel(const Prob *__restrict__ probs,
                                                       const uint32_t *__restrict__ unit_pid,
                                                       const uint32_t *__restrict__ cent, Ptrs<T> io,
                                                       uint32_t *__restrict__ carry_row,
                                                       acc_t<T> *__restrict__ carry_val, uint32_t nunits) {
     using A = acc_t<T>;
-    constexpr int TILE = TPB * IPT;
+    constexpr int TILE = NT * IPT;
     constexpr bool PROD = (K == 1);
     __shared__ uint32_t s_row[TILE];
     __shared__ A s_prod[PROD ? TILE : 1];
     __shared__ uint32_t s_col[PROD ? 1 : TILE];
     __shared__ T s_val[PROD ? 1 : TILE];
-    __shared__ uint32_t s_key[TPB];
-    __shared__ A s_scan[K][TPB];
-    __shared__ A s_wc[K][TPB / WAVE];
+    __shared__ uint32_t s_key[NT];
+    __shared__ A s_scan[K][NT];
+    __shared__ A s_wc[K][NT / WAVE];
     __shared__ T s_x[SLICE ? SLICE_BYTES / (int)sizeof(T) : 1];
 
     const int tid = threadIdx.x;
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     const int ldx = io.lx(P), ldy = io.ly(P);
     if constexpr (SLICE) {
         if (pid != staged) {  // the previous unit ended with a barrier, so s_x is free
-            for (uint32_t c = tid; c < P.ncols_elem; c += TPB) s_x[c] = x[(size_t)c * ldx];
+            for (uint32_t c = tid; c < P.ncols_elem; c += NT) s_x[c] = x[(size_t)c * ldx];
             __syncthreads();
             staged = pid;
         }
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
     T vv[IPT];
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
-        const uint32_t k = tid + q * TPB;
+        const uint32_t k = tid + q * NT;
         if (k < n) {
             rv[q] = ldg(rip + e0 + k);
             cv[q] = ldg(cip + e0 + k);
@@ -613,22 +613,22 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
         T xv[IPT];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < n) xv[q] = SLICE ? s_x[cv[q] >> pb] : x[(size_t)(cv[q] >> pb) * ldx];
+            if (tid + q * NT < n) xv[q] = SLICE ? s_x[cv[q] >> pb] : x[(size_t)(cv[q] >> pb) * ldx];
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < n) s_prod[pb ? (cv[q] & pmask) : tid + q * TPB] = mul<T>(vv[q], xv[q]);
+            if (tid + q * NT < n) s_prod[pb ? (cv[q] & pmask) : tid + q * NT] = mul<T>(vv[q], xv[q]);
     } else {
 #pragma unroll
         for (int q = 0; q < IPT; q++)
-            if (tid + q * TPB < n) {
-                const uint32_t pos = pb ? (cv[q] & pmask) : tid + q * TPB;
+            if (tid + q * NT < n) {
+                const uint32_t pos = pb ? (cv[q] & pmask) : tid + q * NT;
                 s_col[pos] = cv[q] >> pb;
                 s_val[pos] = vv[q];
             }
     }
 #pragma unroll
     for (int q = 0; q < IPT; q++)
-        if (tid + q * TPB < n) s_row[tid + q * TPB] = rv[q];
+        if (tid + q * NT < n) s_row[tid + q * NT] = rv[q];
     __syncthreads();
 
     const uint32_t a = min((uint32_t)tid * IPT, n), b = min(a + IPT, n);
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(TPB) void coo_seg_kernel(const Prob *__restrict__ p
         }
     }
     const uint32_t key = cur;
-    block_seg_scan<A, K>(key, acc, s_key, s_scan, s_wc);
+    block_seg_scan<A, K, NT>(key, acc, s_key, s_scan, s_wc);
     if (emitted) {
         const bool cont = tid > 0 && s_key[tid - 1] == first_row;
 #pragma unroll

@@ -109,7 +109,14 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         switch (w.ipt) {
         case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
         case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
-        case 16: hipLaunchKernelGGL((coo_seg_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+        case 16:
+            if constexpr (K == 1) {
+                // 4096-entry tiles as 512 threads x 8 items (the registers of 8 items per lane)
+                hipLaunchKernelGGL((coo_seg_kernel<T, 8, 1, false, 2 * TPB>), grid, dim3(2 * TPB), 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits);
+            } else {
+                hipLaunchKernelGGL((coo_seg_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits);
+            }
+            break;
         default: return SP_ERR_ARG;
         }
         break;

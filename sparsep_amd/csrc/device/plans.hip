@@ -300,6 +300,10 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
         // no carries); the merge-path kernel for skewed rows, and for large matrices, where
         // its deeper per-thread load batches win (27-point stencil, 113M entries: 1.3x).
         p->algo = (maxlen <= 4 * avg + 64 && A->nnz <= (16u << 20)) ? SP_ALGO_VECTOR : SP_ALGO_SEGMENTED;
+        // Skewed rows and more than 8M entries: 4096-item tiles (16 items per lane, run as
+        // 512 x 8), whose column order shares the most lines (R-MAT 2^20: 77 vs 96 us); below
+        // that there are too few tiles per CU, and regular rows do better with 2048.
+        if (p->opts.items_per_lane == 0 && maxlen > 4 * avg + 64 && A->nnz > (8u << 20)) p->opts.items_per_lane = 16;
         // Column-ordered tiles give the same sums in the same order and gather fewer cache
         // lines (R-MAT 2^20: 8% faster; 27-point stencil: 6%); they need col << log2(tile)
         // to fit 32 bits.
@@ -442,7 +446,13 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
         P.nrows = P.nrows_elem = n;
         P.nent = nnz;
         P.ncols_elem = A->ncols;
-        const uint32_t I = pick_ipt(p->opts, 4);  // 4 items per lane: the tuner's usual winner for 1D COO
+        // 4 items per lane, the tuner's usual winner for 1D COO; skewed rows and more than 8M
+        // entries take 4096-entry tiles (16, run as 512 x 8: R-MAT 2^20 86 vs 109 us), as for CSR
+        uint32_t maxlen = 0;
+        for (uint32_t i = 0; i < n; i++) maxlen = std::max(maxlen, rowptr[i + 1] - rowptr[i]);
+        if (p->opts.items_per_lane == 0 && maxlen > 4.0 * nnz / std::max(n, 1u) + 64 && nnz > (8u << 20))
+            p->opts.items_per_lane = 16;
+        const uint32_t I = pick_ipt(p->opts, 4);
         if (!I) { rc = SP_ERR_ARG; break; }
         p->ipt = p->work.ipt = I;
         const uint32_t tile = TPB * I;
