@@ -849,19 +849,21 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             if ((rc = sp_partition_tiled(T, horz, tk.by_nnz, split.data()))) break;
         }
         int kind;
+        bool col_order = false;  // column-ordered tiles (CSR merge path, COO), as for 1D
         if (tk.format == SP_FMT_CSR) {
             if (p->algo == SP_ALGO_AUTO) {
                 const double avg = n ? (double)A0->nnz / n : 0.0;
                 // as for 1D CSR; stripes also leave row ranges empty, which the merge path
                 // skips at no cost
                 p->algo = (maxlen <= 4 * avg + 64 && A0->nnz <= (16u << 20)) ? SP_ALGO_VECTOR
-                                                                             : SP_ALGO_SEGMENTED;
+                                                                             : SP_ALGO_SEGMENTED_SORTED;
             }
-            if (p->algo == SP_ALGO_SEGMENTED_SORTED || p->algo == SP_ALGO_SEGMENTED_HOT)
-                p->algo = SP_ALGO_SEGMENTED;  // 1D CSR only
+            if (p->algo == SP_ALGO_SEGMENTED_HOT) p->algo = SP_ALGO_SEGMENTED_SORTED;  // hot x: 1D CSR only
             kind = p->algo == SP_ALGO_VECTOR ? WK_CSR_VEC : WK_CSR_MERGE;
+            col_order = p->algo == SP_ALGO_SEGMENTED_SORTED;
         } else if (tk.format == SP_FMT_COO) {
             kind = WK_COO_SEG;
+            col_order = p->algo != SP_ALGO_SEGMENTED;  // as for 1D COO
             p->algo = SP_ALGO_SEGMENTED;
         } else if (tk.format == SP_FMT_BCSR) {
             kind = WK_BCSR_VEC;
@@ -903,6 +905,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
         p->work.R = R;
         p->work.C = C;
         const uint32_t tile_items = TPB * p->work.ipt;
+        std::vector<uint32_t> order_cuts;  // column-ordered tiles: unit cuts over all entries
         for (uint32_t t = 0; t < nt && !rc; t++) {
             const uint32_t c = t % vert;
             const uint32_t row_off = tk.tiling == SP_TILE_D ? (t / vert) * th : 0;
@@ -1001,9 +1004,41 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
                     }
                 }
             }
+            if (col_order && !blk) {  // the tile's units as absolute entry cuts
+                const std::vector<uint32_t> &cuts = kind == WK_CSR_MERGE ? k1 : k0;
+                const uint64_t e0 = T->nnz_offset[t];
+                for (size_t u = 0; u < cuts.size(); u++) {
+                    if (u + 1 < cuts.size() && cuts[u + 1] - cuts[u] > tile_items) col_order = false;
+                    push_cut(order_cuts, (uint32_t)(e0 + cuts[u]));
+                }
+            }
             b.add(P, k0, kind == WK_CSR_MERGE ? &k1 : nullptr);
         }
         if (rc) break;
+        // Column-ordered tiles (as for 1D CSR/COO): every unit's entries sorted by local column,
+        // with the row-order position in the low bits; the arrays uploaded above are replaced.
+        if (col_order && !blk && T->nnz) {
+            uint32_t pb = 0;
+            while ((1u << pb) < tile_items) pb++;
+            if (A0->ncols <= (1ull << (32 - pb)) && !order_cuts.empty() && order_cuts.front() == 0 &&
+                order_cuts.back() == T->nnz) {
+                std::vector<uint32_t> key;
+                std::vector<char> val;
+                column_ordered_tiles(T->dcolind, T->dval, (uint32_t)T->nnz, order_cuts, pb, sv, key, val);
+                if (hipMemcpy((void *)d_cind, key.data(), (size_t)T->nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy((void *)d_val, val.data(), (size_t)T->nnz * sv, hipMemcpyHostToDevice) != hipSuccess) {
+                    rc = SP_ERR_HIP;
+                    break;
+                }
+                for (Prob &Q : b.probs) Q.pbits = pb;
+            } else {
+                col_order = false;
+            }
+        }
+        if (tk.format == SP_FMT_CSR && kind == WK_CSR_MERGE)
+            p->algo = col_order ? SP_ALGO_SEGMENTED_SORTED : SP_ALGO_SEGMENTED;
+        else if (tk.format == SP_FMT_COO)
+            p->algo = col_order ? SP_ALGO_SEGMENTED_SORTED : SP_ALGO_SEGMENTED;
         // algorithmic bytes: the tile formats' arrays, x once and y once (the partial
         // vectors and their reduction are overhead, not algorithm)
         {

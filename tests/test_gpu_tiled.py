@@ -91,3 +91,19 @@ def test_lds_x_slices(sp, ctx, dt):
             assert_parity_multi(dt, Y1, A, X, f"{kern}/{kw}/slice spmm")
     with pytest.raises(sp._lib.SparsePError):
         sp.Plan(ctx, "RBDCSR", A, sp.opts(vert=2, lds_slice=A.ncols // 4))
+
+
+@pytest.mark.parametrize("dt", [2, 4, 5])
+def test_2d_column_order_same_bits(sp, ctx, dt):
+    """2D CSR (merge path) and COO tiles in column order (the default where the
+    columns fit) give the same bits as row-ordered tiles (algo=2)."""
+    A = sp.gen_rmat(15, 400000, dtype=dt, seed=81)
+    x = sp.gen_vector(dt, A.ncols, seed=82)
+    for kern in ("DCSR", "RBDCSR", "BDCSR", "DCOO", "RBDCOO", "BDCOO"):
+        for kw in (dict(vert=2), dict(vert=4, nparts=32, nsub=4), dict(vert=3, items_per_lane=16)):
+            extra = dict(algo=3) if "CSR" in kern else {}   # CSR: column-ordered merge tiles
+            y0, _ = run(sp, ctx, kern, A, x, sp.opts(algo=2, **kw))
+            y1, plan = run(sp, ctx, kern, A, x, sp.opts(**extra, **kw))
+            assert plan.info()["algo"] == 3, (kern, kw)
+            assert_parity(dt, y1, A, x, f"{kern}/{kw}")
+            assert np.array_equal(y0, y1), f"{kern}/{kw}"
