@@ -802,9 +802,9 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
     sp_tiled *T = nullptr;
     int rc = sp_coo2tiled(A, tk.tiling, tk.format, horz, vert, p->opts.row_block_size, p->opts.col_block_size,
                           p->opts.count_values, &T);
-    // row statistics for the CSR algorithm choice
+    // row statistics for the CSR algorithm and the tile size choice
     uint32_t maxlen = 0;
-    if (!rc && tk.format == SP_FMT_CSR && p->algo == SP_ALGO_AUTO) {
+    if (!rc && (tk.format == SP_FMT_CSR || tk.format == SP_FMT_COO)) {
         std::vector<uint32_t> cnt(n + 1, 0);
         for (uint32_t k = 0; k < A->nnz; k++) cnt[A->rowind[k]]++;
         for (uint32_t i = 0; i < n; i++) maxlen = std::max(maxlen, cnt[i]);
@@ -872,6 +872,10 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             kind = WK_BCOO_SEG;
             p->algo = SP_ALGO_SEGMENTED;
         }
+        // skewed rows and more than 8M entries: 4096-entry tiles, as for 1D CSR/COO
+        if ((kind == WK_CSR_MERGE || kind == WK_COO_SEG) && p->opts.items_per_lane == 0 && !p->opts.lds_slice &&
+            maxlen > 4.0 * A0->nnz / std::max(n, 1u) + 64 && A0->nnz > (8u << 20))
+            p->opts.items_per_lane = 16;
         const uint32_t I = pick_ipt(p->opts, 8);
         if (!I) { rc = SP_ERR_ARG; break; }
         // x slices staged in LDS (2D CSR/COO, the reference's per-DPU x slice,
