@@ -303,7 +303,10 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
         // Skewed rows and more than 8M entries: 4096-item tiles (16 items per lane, run as
         // 512 x 8), whose column order shares the most lines (R-MAT 2^20: 77 vs 96 us); below
         // that there are too few tiles per CU, and regular rows do better with 2048.
-        if (p->opts.items_per_lane == 0 && maxlen > 4 * avg + 64 && A->nnz > (8u << 20)) p->opts.items_per_lane = 16;
+        // (only where they can be column ordered, ncols <= 2^20: row-ordered 4096-item tiles
+        // are slower than column-ordered 2048-item ones)
+        if (p->opts.items_per_lane == 0 && maxlen > 4 * avg + 64 && A->nnz > (8u << 20) && A->ncols <= (1u << 20))
+            p->opts.items_per_lane = 16;
         // Column-ordered tiles give the same sums in the same order and gather fewer cache
         // lines (R-MAT 2^20: 8% faster; 27-point stencil: 6%); they need col << log2(tile)
         // to fit 32 bits.
@@ -450,7 +453,8 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
         // entries take 4096-entry tiles (16, run as 512 x 8: R-MAT 2^20 86 vs 109 us), as for CSR
         uint32_t maxlen = 0;
         for (uint32_t i = 0; i < n; i++) maxlen = std::max(maxlen, rowptr[i + 1] - rowptr[i]);
-        if (p->opts.items_per_lane == 0 && maxlen > 4.0 * nnz / std::max(n, 1u) + 64 && nnz > (8u << 20))
+        if (p->opts.items_per_lane == 0 && maxlen > 4.0 * nnz / std::max(n, 1u) + 64 && nnz > (8u << 20) &&
+            A->ncols <= (1u << 20) && p->algo != SP_ALGO_SEGMENTED)  // column-ordered 4096-entry tiles only
             p->opts.items_per_lane = 16;
         const uint32_t I = pick_ipt(p->opts, 4);
         if (!I) { rc = SP_ERR_ARG; break; }
@@ -873,8 +877,9 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             p->algo = SP_ALGO_SEGMENTED;
         }
         // skewed rows and more than 8M entries: 4096-entry tiles, as for 1D CSR/COO
-        if ((kind == WK_CSR_MERGE || kind == WK_COO_SEG) && p->opts.items_per_lane == 0 && !p->opts.lds_slice &&
-            maxlen > 4.0 * A0->nnz / std::max(n, 1u) + 64 && A0->nnz > (8u << 20))
+        if ((kind == WK_CSR_MERGE || kind == WK_COO_SEG) && col_order && p->opts.items_per_lane == 0 &&
+            !p->opts.lds_slice && maxlen > 4.0 * A0->nnz / std::max(n, 1u) + 64 && A0->nnz > (8u << 20) &&
+            A0->ncols <= (1u << 20))
             p->opts.items_per_lane = 16;
         const uint32_t I = pick_ipt(p->opts, 8);
         if (!I) { rc = SP_ERR_ARG; break; }
