@@ -1234,4 +1234,37 @@ __global__ void fixup_kernel(const Prob *__restrict__ probs, const uint32_t *__r
     for (int q = 0; q < K; q++) yp[q] = T(0);
 }
 
+// ---------------------------------------------------------------------------
+// CSR merge fix-up from a list of row runs.  Which rows cross tiles depends
+// only on the tiling (tile t carries the row at its end coordinate), so the
+// plan lists the runs once: one thread per run adds the run's carries in
+// tile order to the row's y, as fixup_kernel does.  Every load a thread needs
+// goes out after its one run record arrives (table launches add ytab[pid]).
+// ---------------------------------------------------------------------------
+struct Run {
+    uint32_t pid, t0, n, row;  // problem, first unit, units, row carried
+    int64_t yel;               // P.yoff + row: y element of the row (no pointer tables)
+    int32_t ldy;               // P.ldy (pointer tables)
+    uint32_t pad;
+};
+
+template <typename T, int K>
+__global__ void run_fixup_kernel(const Run *__restrict__ runs, uint32_t nruns, const acc_t<T> *__restrict__ carry_val,
+                                 Ptrs<T> io) {
+    using A = acc_t<T>;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nruns) return;
+    const Run R = runs[j];
+    const gptr<T> yp = io.ytab ? gp((T *)io.ytab[R.pid] + io.q0) + (size_t)R.row * R.ldy : gp(io.yb) + R.yel * io.ldy;
+    A sum[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) sum[q] = A(0) + carry_val[(size_t)R.t0 * K + q];
+    for (uint32_t u = 1; u < R.n; u++) {
+#pragma unroll
+        for (int q = 0; q < K; q++) sum[q] = sum[q] + carry_val[(size_t)(R.t0 + u) * K + q];
+    }
+#pragma unroll
+    for (int q = 0; q < K; q++) yp[q] = to_val<T>((A)yp[q] + sum[q]);
+}
+
 }  // namespace spd

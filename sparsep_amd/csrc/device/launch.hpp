@@ -188,6 +188,15 @@ static int launch_chunk(const Work &w, sp_ctx *ctx, Ptrs<T> io) {
         if (rc) return rc;
     }
     mark_dominant(ctx);
+    if (w.d_runs) {  // CSR merge with its run list
+        if (w.nruns) {
+            using A = acc_t<T>;
+            hipLaunchKernelGGL((run_fixup_kernel<T, K>), dim3((w.nruns + 255) / 256), dim3(256), 0, s, w.d_runs, w.nruns,
+                               (const A *)w.d_carry_val, io);
+            if (hipGetLastError() != hipSuccess) return SP_ERR_HIP;
+        }
+        return SP_OK;
+    }
     const bool carries = w.kind == WK_CSR_MERGE || w.kind == WK_COO_SEG || w.kind == WK_BCOO_SEG || w.kind == WK_CSR_HOT ||
                          w.kind == WK_BCSR_SEG;
     const uint32_t S = (w.kind == WK_BCOO_SEG || w.kind == WK_BCSR_SEG) ? w.R : 1;

@@ -69,7 +69,26 @@ struct Builder {
             if ((rc = plan_alloc(p, n * 4, (void **)&w.d_carry_row))) return rc;
             if ((rc = plan_alloc(p, n * w.carry_k * acc_size(p->dtype), &w.d_carry_val))) return rc;
         }
-        p->launches = 1 + ((carries || w.nzero) ? 1 : 0);
+        if (kind == WK_CSR_MERGE && carries && !w.nzero) {
+            // tile t carries row c0[t + pid + 1] when that row exists; consecutive
+            // tiles carrying the same row form one run
+            std::vector<spd::Run> runs;
+            for (uint32_t t = 0; t < w.nunits; t++) {
+                const uint32_t pid = unit_pid[t], r = c0[(size_t)t + pid + 1];
+                const Prob &P = probs[pid];
+                if (r >= P.nrows || r >= P.nrows_elem) continue;
+                if (!runs.empty()) {
+                    spd::Run &b = runs.back();
+                    if (b.pid == pid && b.row == r && b.t0 + b.n == t) { b.n++; continue; }
+                }
+                runs.push_back(spd::Run{pid, t, 1, r, P.yoff + (int64_t)r, P.ldy, 0});
+            }
+            w.nruns = (uint32_t)runs.size();
+            if ((rc = plan_upload(p, runs.data(), runs.size() * sizeof(spd::Run), (void **)&w.d_runs))) return rc;
+            p->launches = 1 + (w.nruns ? 1 : 0);
+        } else {
+            p->launches = 1 + ((carries || w.nzero) ? 1 : 0);
+        }
         p->h_probs = probs;
         p->h_unit_pid = unit_pid;
         p->h_c0 = c0;

@@ -33,6 +33,8 @@ struct Work {
     uint64_t zero_rows = 0; // rows to zero when zero_all (times ldy elements)
     uint32_t grid_cap = 0;  // persistent launch: at most this many workgroups (0 = one per unit)
     bool slice = false;     // 2D CSR/COO tiles: one right-hand side gathers from an LDS copy of the tile's x slice
+    spd::Run *d_runs = nullptr;  // CSR merge: row runs that cross tiles (fix-up list), nruns of them
+    uint32_t nruns = 0;
 };
 
 // Widest right-hand-side chunk a kernel kind takes in one pass: the
