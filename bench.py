@@ -197,21 +197,23 @@ def _rmat_all(sp, dt):
 def wl_formats(sp, ctx, args, orc):
     import numpy as np
     out = []
-    kinds = ["CSR-row", "CSR-nnz", "COO-row", "COO-nnz-rgrn", "COO-nnz", "BCSR-block", "BCSR-nnz", "BCOO-block",
-             "BCOO-nnz"]
+    # (kernel, algo): default plans, then the atomic merges (algo 5) of COO-nnz and BCOO-nnz
+    kinds = [(k, 0) for k in ("CSR-row", "CSR-nnz", "COO-row", "COO-nnz-rgrn", "COO-nnz", "BCSR-block", "BCSR-nnz",
+                              "BCOO-block", "BCOO-nnz")] + [("COO-nnz", 5), ("BCOO-nnz", 5)]
     for dt in args.dtypes:
         A, vals = _rmat_all(sp, dt)
         x = sp.gen_vector(dt, A.ncols, d=vals, seed=22)
         xd = sp.DeviceArray.from_numpy(ctx, x)
         yd = sp.DeviceArray(ctx, A.nrows, x.dtype)
-        for kname in kinds:
-            plan = sp.Plan(ctx, kname, A)
+        for kbase, algo in kinds:
+            kname = kbase + (" atomic" if algo else "")
+            plan = sp.Plan(ctx, kbase, A, sp.opts(algo=algo))
             info = plan.info()
             mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
             ok = parity_ok(orc, dt, A, x, yd.numpy())
             gbs = info["algo_bytes"] / (mean_dom * 1e-3) / 1e9
             log(f"[bench] formats {DT_NAME[dt]} {kname:13s} {mean_ms * 1e3:8.2f} us  {gbs:7.0f} GB/s  parity {ok}")
-            cfg = {"workload": f"R-MAT 2^20, 2^24 nnz, {kname}, MALL flushed", "kernel": kname,
+            cfg = {"workload": f"R-MAT 2^20, 2^24 nnz, {kname}, MALL flushed", "kernel": kbase, "algo": info["algo"],
                    "nnz": A.nnz, "algo_bytes": info["algo_bytes"]}
             out.append(line(args, info["algo_bytes"] / (mean_ms * 1e-3) / 1e9, mean_ms, DT_NAME[dt], cfg,
                             roofline={"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,

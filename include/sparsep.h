@@ -410,7 +410,19 @@ typedef enum sp_algo {
      * LDS (up to 112 KB; 96 KB for 8-byte accumulators): one 1024-thread
      * workgroup per CU, tiles of 4096 merge items.  CSR ids 0 and 1 only;
      * needs ncols <= 2^20.  Several right-hand sides run one at a time. */
-    SP_ALGO_SEGMENTED_HOT = 4
+    SP_ALGO_SEGMENTED_HOT = 4,
+    /* SEGMENTED_SORTED with the rows that cross tiles merged by device
+     * atomics instead of carries: the reference's CGLOCK / FGLOCK builds
+     * (spmv/1D/COO-nnz/Makefile:7-9: one or 32 hashed mutexes around
+     * y[row] += partial, spmv/1D/COO-nnz/dpu/task.c:819-885; likewise
+     * spmv/1D/BCOO-nnz/dpu/task.c).  Its LOCKFREE default is the carry
+     * merge of SEGMENTED / SEGMENTED_SORTED.  1D COO
+     * (ids 2-4) and BCOO (ids 7, 8); other ids return SP_ERR_UNSUPPORTED.
+     * Integer results are exact; floating-point sums of a row spread over
+     * three or more tiles can differ in the last bits between runs.  Rows
+     * the atomics touch are zeroed by a small kernel before the main one.
+     * Batched launches (sp_batch_*) merge these plans with carries. */
+    SP_ALGO_SEGMENTED_ATOMIC = 5
 } sp_algo;
 
 /* Tasklet-level balance (the reference BLNC_TSKLT_* knobs). */

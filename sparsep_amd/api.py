@@ -21,7 +21,7 @@ KERNELS = ["CSR-row", "CSR-nnz", "COO-row", "COO-nnz-rgrn", "COO-nnz", "BCSR-blo
            "BDBCSR-block", "BDBCSR-nnz", "BDBCOO-block", "BDBCOO-nnz"]
 KID = {n: i for i, n in enumerate(KERNELS)}
 
-ALGO_AUTO, ALGO_VECTOR, ALGO_SEGMENTED, ALGO_SEGMENTED_SORTED, ALGO_SEGMENTED_HOT = 0, 1, 2, 3, 4
+ALGO_AUTO, ALGO_VECTOR, ALGO_SEGMENTED, ALGO_SEGMENTED_SORTED, ALGO_SEGMENTED_HOT, ALGO_SEGMENTED_ATOMIC = 0, 1, 2, 3, 4, 5
 BAL_ROW, BAL_NNZ, BAL_BLOCK = 0, 1, 2
 TILE_D, TILE_RB, TILE_BD = 0, 1, 2
 FMT_CSR, FMT_COO, FMT_BCSR, FMT_BCOO = 0, 1, 2, 3

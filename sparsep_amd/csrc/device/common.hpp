@@ -148,4 +148,31 @@ template <typename T, int N> __device__ __forceinline__ void ld_n(gptr<const T> 
     for (int i = 0; i < N; i++) out[i] = p[i];
 }
 
+// y[0] += v as one device-scope atomic: the lock-based merges of the
+// reference's COO/BCOO kernels (one mutex, or 32 hashed mutexes, around
+// `y[row] += partial`, spmv/1D/COO-nnz/dpu/task.c:819-885).  4- and 8-byte
+// values use a hardware add (integers modulo 2^bits, as the accumulation);
+// 1- and 2-byte integers a compare-and-swap on the aligned 32-bit word that
+// holds them, so bytes next to y[0] are written back unchanged.
+template <typename T> __device__ __forceinline__ void atomic_add_y(gptr<T> p, acc_t<T> v) {
+    if constexpr (__is_same(T, float) || __is_same(T, double)) {
+        __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if constexpr (sizeof(T) == 4) {
+        __hip_atomic_fetch_add((gptr<uint32_t>)p, (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if constexpr (sizeof(T) == 8) {
+        __hip_atomic_fetch_add((gptr<uint64_t>)p, (uint64_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        const uintptr_t a = (uintptr_t)p;
+        const gptr<uint32_t> w = (gptr<uint32_t>)(a & ~(uintptr_t)3);
+        const uint32_t sh = (uint32_t)(a & 3) * 8, m = (sizeof(T) == 1 ? 0xFFu : 0xFFFFu) << sh;
+        uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (;;) {
+            const uint32_t nw = (old & ~m) | ((((old & m) >> sh) + (uint32_t)v) << sh & m);
+            if (__hip_atomic_compare_exchange_strong(w, &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+                break;
+        }
+    }
+}
+
 }  // namespace spd

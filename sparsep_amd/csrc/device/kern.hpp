@@ -550,7 +550,7 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
 // thread holding its last entry, with the partial sums of earlier threads
 // of the tile added in; partials of earlier tiles arrive in the fix-up.
 // ---------------------------------------------------------------------------
-template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB>
+template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB, bool ATOM = false>
 __global__ __launch_bounds__(NT) void coo_seg_kernel(const Prob *__restrict__ probs,
                                                       const uint32_t *__restrict__ unit_pid,
                                                       const uint32_t *__restrict__ cent, Ptrs<T> io,
@@ -683,19 +683,29 @@ __global__ __launch_bounds__(NT) void coo_seg_kernel(const Prob *__restrict__ pr
     }
     const uint32_t key = cur;
     block_seg_scan<A, K, NT>(key, acc, s_key, s_scan, s_wc);
+    // ATOM (the reference's lock-based merges): the tile's first row, which may
+    // have begun in an earlier tile, and the row it carries out are added to y
+    // with atomics (both rows are zeroed before the launch); no carries.
     if (emitted) {
         const bool cont = tid > 0 && s_key[tid - 1] == first_row;
-#pragma unroll
+        const bool shared = ATOM && first_row == s_row[0];
         for (int q = 0; q < K; q++) {
             const A cin = cont ? s_scan[q][tid - 1] : A(0);
-            y[(size_t)first_row * ldy + q] = to_val<T>(cin + first[q]);
+            if (shared) atomic_add_y<T>(y + (size_t)first_row * ldy + q, cin + first[q]);
+            else y[(size_t)first_row * ldy + q] = to_val<T>(cin + first[q]);
         }
     }
     const uint32_t last = n ? (n - 1) / IPT : 0;
     if ((uint32_t)tid == last) {
-        carry_row[t] = (n == 0 || final_emitted) ? NONE : key;
+        if constexpr (ATOM) {
+            if (n && !final_emitted) {
+                for (int q = 0; q < K; q++) atomic_add_y<T>(y + (size_t)key * ldy + q, acc[q]);
+            }
+        } else {
+            carry_row[t] = (n == 0 || final_emitted) ? NONE : key;
 #pragma unroll
-        for (int q = 0; q < K; q++) carry_val[(size_t)t * K + q] = acc[q];
+            for (int q = 0; q < K; q++) carry_val[(size_t)t * K + q] = acc[q];
+        }
     }
     __syncthreads();  // LDS is reused by the next tile
     }
@@ -980,7 +990,7 @@ __global__ __launch_bounds__(TPB) void rows_rhs_kernel(const Prob *__restrict__ 
 // LDS instead of a stored row index, and a tile's last row is complete when
 // its end lies in the tile.  Block rows without blocks are zeroed by the
 // fix-up.
-template <typename T, int R, int C, int IPT, bool PTR = false>
+template <typename T, int R, int C, int IPT, bool PTR = false, bool ATOM = false>
 __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ probs,
                                                        const uint32_t *__restrict__ unit_pid,
                                                        const uint32_t *__restrict__ cent,
@@ -1164,20 +1174,35 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
     }
     const uint32_t key = cur;
     block_seg_scan<A, R>(key, acc, s_key, s_scan, s_wc);
+    // ATOM (BCOO only, the reference's lock-based merges): the tile's first
+    // block row and the block row it carries out are added to y with atomics
+    // (their rows are zeroed before the launch); no carries.
+    auto add_block_row = [&](uint32_t brow, const A (&v)[R]) {
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const uint32_t row = brow * R + i;
+            if (row < P.nrows_elem) atomic_add_y<T>(y + (size_t)row * ldy, v[i]);
+        }
+    };
     if (emitted) {
         const bool cont = tid > 0 && s_key[tid - 1] == first_row;
         A v[R];
 #pragma unroll
         for (int i = 0; i < R; i++) v[i] = (cont ? s_scan[i][tid - 1] : A(0)) + first[i];
-        emit_block_row(first_row, v);
+        if (ATOM && first_row == s_brow[0]) add_block_row(first_row, v);
+        else emit_block_row(first_row, v);
     }
     const uint32_t last = n ? (n - 1) / IPT : 0;
     if ((uint32_t)tid == last) {
-        const uint32_t kr = (n == 0 || final_emitted) ? NONE : key;
+        if constexpr (ATOM) {
+            if (n && !final_emitted) add_block_row(key, acc);
+        } else {
+            const uint32_t kr = (n == 0 || final_emitted) ? NONE : key;
 #pragma unroll
-        for (int i = 0; i < R; i++) {
-            carry_row[(size_t)i * nunits + t] = kr;
-            carry_val[(size_t)i * nunits + t] = acc[i];
+            for (int i = 0; i < R; i++) {
+                carry_row[(size_t)i * nunits + t] = kr;
+                carry_val[(size_t)i * nunits + t] = acc[i];
+            }
         }
     }
     __syncthreads();  // LDS is reused by the next tile

@@ -96,6 +96,20 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
 #undef SPD_VEC
         break;
     case WK_COO_SEG:
+        if (w.atomic) {  // 1D plans only: no slices
+            switch (w.ipt) {
+            case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, K, false, TPB, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+            case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, K, false, TPB, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+            case 16:
+                if constexpr (K == 1)
+                    hipLaunchKernelGGL((coo_seg_kernel<T, 8, 1, false, 2 * TPB, true>), grid, dim3(2 * TPB), 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits);
+                else
+                    hipLaunchKernelGGL((coo_seg_kernel<T, 16, K, false, TPB, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits);
+                break;
+            default: return SP_ERR_ARG;
+            }
+            break;
+        }
         if constexpr (K == 1) {
             if (w.slice) {
                 switch (w.ipt) {
@@ -163,6 +177,10 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         if (K != 1) return SP_ERR_UNSUPPORTED;
 #define SPD_BO(Rv, Cv) \
     if (w.R == Rv && w.C == Cv) { \
+        if (w.atomic) { \
+            if (w.ipt == 2) hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 2, false, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, nullptr, io, w.d_carry_row, cv, w.nunits); \
+            else hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 4, false, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, nullptr, io, w.d_carry_row, cv, w.nunits); \
+            break; } \
         if (w.ipt == 2) hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 2>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, nullptr, io, w.d_carry_row, cv, w.nunits); \
         else hipLaunchKernelGGL((bcoo_seg_kernel<T, Rv, Cv, 4>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, nullptr, io, w.d_carry_row, cv, w.nunits); \
         break; }
@@ -182,6 +200,23 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
 template <typename T, int K>
 static int launch_chunk(const Work &w, sp_ctx *ctx, Ptrs<T> io) {
     hipStream_t s = ctx->stream;
+    if (w.atomic) {
+        // rows shared between tiles (and rows without entries) start at zero; the main
+        // kernel adds the shared rows' partial sums atomically, so nothing follows it
+        using A = acc_t<T>;
+        if (w.nprez) {
+            hipLaunchKernelGGL((fixup_kernel<T, K>), dim3((w.nprez + 255) / 256), dim3(256), 0, s, w.d_prob,
+                               w.d_unit_pid, w.d_carry_row, (const A *)w.d_carry_val, 0u, 1u, io, w.d_prez_pid,
+                               w.d_prez_row, w.nprez);
+            if (hipGetLastError() != hipSuccess) return SP_ERR_HIP;
+        }
+        if (w.nunits) {
+            const int rc = launch_main<T, K>(w, s, io);
+            if (rc) return rc;
+        }
+        mark_dominant(ctx);
+        return SP_OK;
+    }
     if (w.nunits == 0 && w.nzero == 0) return SP_OK;
     if (w.nunits) {
         int rc = launch_main<T, K>(w, s, io);

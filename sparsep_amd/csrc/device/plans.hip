@@ -217,6 +217,32 @@ void column_ordered_tiles(const uint32_t *colind, const void *values, uint32_t n
     for (auto &t : th) t.join();
 }
 
+// Atomic merges (SP_ALGO_SEGMENTED_ATOMIC): the rows a tile may share with
+// its neighbours, those of its first and last entry (R element rows each for
+// blocks), are zeroed before the launch, together with the listed rows that
+// have no entries; the kernel then adds those rows' partial sums atomically.
+int atomic_prezero(sp_plan *p, const uint32_t *rind, const std::vector<uint32_t> &units, uint32_t R,
+                   uint32_t nrows_elem, const std::vector<uint32_t> &zrow) {
+    std::vector<uint32_t> rows(zrow);
+    for (size_t u = 0; u + 1 < units.size(); u++) {
+        if (units[u + 1] == units[u]) continue;
+        for (const uint32_t r : {rind[units[u]], rind[units[u + 1] - 1]})
+            for (uint32_t i = 0; i < R; i++)
+                if ((uint64_t)r * R + i < nrows_elem) rows.push_back(r * R + i);
+    }
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    Work &w = p->work;
+    w.atomic = true;
+    w.nprez = (uint32_t)rows.size();
+    const std::vector<uint32_t> pid(rows.size(), 0);
+    int rc;
+    if ((rc = plan_upload(p, pid.data(), pid.size() * 4, (void **)&w.d_prez_pid))) return rc;
+    if ((rc = plan_upload(p, rows.data(), rows.size() * 4, (void **)&w.d_prez_row))) return rc;
+    p->launches = 1 + (w.nprez ? 1 : 0);
+    return SP_OK;
+}
+
 // Blocks per lane of the BCOO kernel: its LDS holds R sums per block of the
 // tile, so tall blocks with 8-byte sums take half the tile (FP64 4x4: 50 KB ->
 // 26 KB of LDS, 3 -> 6 workgroups per CU).
@@ -464,6 +490,7 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
     int rc = SP_OK;
     void *d;
     Builder b(p);
+    const bool atom = p->algo == SP_ALGO_SEGMENTED_ATOMIC;
     do {
         P.nrows = P.nrows_elem = n;
         P.nent = nnz;
@@ -528,7 +555,7 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
         if (nnz == 0) units.clear();
         for (size_t u = 0; u + 1 < units.size(); u++)
             if (units[u + 1] - units[u] > tile) col_order = false;  // positions must fit pb bits
-        p->algo = col_order ? SP_ALGO_SEGMENTED_SORTED : SP_ALGO_SEGMENTED;
+        p->algo = atom ? SP_ALGO_SEGMENTED_ATOMIC : col_order ? SP_ALGO_SEGMENTED_SORTED : SP_ALGO_SEGMENTED;
         if ((rc = plan_upload(p, A->rowind, (size_t)nnz * 4, &d))) break;
         P.rind = (const uint32_t *)d;
         if (col_order) {
@@ -552,6 +579,7 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
             if (rowptr[i + 1] == rowptr[i]) { b.zpid.push_back(0); b.zrow.push_back(i); }
         b.add(P, units, nullptr);
         rc = b.finish(WK_COO_SEG, true, 1);
+        if (!rc && atom) rc = atomic_prezero(p, A->rowind, units, 1, n, b.zrow);
     } while (0);
     if (sorted) sp_free_coo(sorted);
     return rc;
@@ -680,7 +708,8 @@ int build_bcoo_plan(sp_plan *p, const sp_bcoo *B) {
     P.nent = (uint32_t)nb;
     P.nrows_elem = B->nrows;
     P.ncols_elem = B->ncols;
-    p->algo = SP_ALGO_SEGMENTED;
+    const bool atom = p->algo == SP_ALGO_SEGMENTED_ATOMIC;
+    p->algo = atom ? SP_ALGO_SEGMENTED_ATOMIC : SP_ALGO_SEGMENTED;
     p->work.R = R;
     p->work.C = C;
     p->ipt = p->work.ipt = bcoo_ipt(B->dtype, R);
@@ -721,7 +750,8 @@ int build_bcoo_plan(sp_plan *p, const sp_bcoo *B) {
                 b.zrow.push_back(I * R + i);
             }
     b.add(P, units, nullptr);
-    return b.finish(WK_BCOO_SEG, true, R);
+    if ((rc = b.finish(WK_BCOO_SEG, true, R))) return rc;
+    return atom ? atomic_prezero(p, B->browind, units, R, B->nrows, b.zrow) : SP_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -173,9 +173,19 @@ static sp_plan *new_plan(sp_ctx *ctx, int id, int dtype, const sp_opts *opts) {
     return p;
 }
 
+// The algorithm asked for exists and applies to the kernel id.
+static int check_algo(int id, const sp_opts *opts) {
+    const int algo = opts ? opts->algo : SP_ALGO_AUTO;
+    if (algo < SP_ALGO_AUTO || algo > SP_ALGO_SEGMENTED_ATOMIC) return SP_ERR_ARG;
+    if (algo == SP_ALGO_SEGMENTED_ATOMIC && !is_coo(id) && id != SP_K_BCOO_BLOCK && id != SP_K_BCOO_NNZ)
+        return SP_ERR_UNSUPPORTED;
+    return SP_OK;
+}
+
 extern "C" int sp_plan_create_csr(sp_ctx *ctx, int id, const sp_csr *A, const sp_opts *opts,
                                   sp_plan **out) {
     if (!ctx || !A || !out || !is_csr(id)) return SP_ERR_ARG;
+    if (int rc = check_algo(id, opts)) return rc;
     if (sp_dtype_size(A->dtype) == 0) return SP_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     sp_plan *p = new_plan(ctx, id, A->dtype, opts);
@@ -191,6 +201,7 @@ extern "C" int sp_plan_create_csr(sp_ctx *ctx, int id, const sp_csr *A, const sp
 
 extern "C" int sp_plan_create(sp_ctx *ctx, int id, const sp_coo *A, const sp_opts *opts, sp_plan **out) {
     if (!ctx || !A || !out || id < 0 || id >= SP_K_COUNT) return SP_ERR_ARG;
+    if (int rc = check_algo(id, opts)) return rc;
     if (sp_dtype_size(A->dtype) == 0) return SP_ERR_ARG;
     if (A->nnz && (!A->rowind || !A->colind || !A->values)) return SP_ERR_ARG;
     for (uint32_t k = 0; k < A->nnz; k++)

@@ -35,6 +35,9 @@ struct Work {
     bool slice = false;     // 2D CSR/COO tiles: one right-hand side gathers from an LDS copy of the tile's x slice
     spd::Run *d_runs = nullptr;  // CSR merge: row runs that cross tiles (fix-up list), nruns of them
     uint32_t nruns = 0;
+    bool atomic = false;         // COO/BCOO: rows crossing tiles merged by atomics (no carries)
+    uint32_t nprez = 0;          // atomic: rows zeroed before the main launch
+    uint32_t *d_prez_pid = nullptr, *d_prez_row = nullptr;
 };
 
 // Widest right-hand-side chunk a kernel kind takes in one pass: the

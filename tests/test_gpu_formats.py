@@ -171,3 +171,50 @@ def test_bcsr_segmented(sp, ctx, rc, dt):
     # the hub matrix is skewed: the default picks the entry-balanced kernel
     _, plan = run(sp, ctx, "BCSR-nnz", hub, sp.gen_vector(dt, hub.ncols, seed=9), sp.opts(row_block_size=r, col_block_size=c))
     assert plan.info()["algo"] == 2
+
+
+@pytest.mark.parametrize("dt", [0, 1, 2, 3, 4, 5])
+def test_atomic_merges(sp, ctx, dt):
+    """algo=5: COO and BCOO plans whose rows crossing tiles are added to y with
+    device atomics (the reference's lock-based cg/fg merges) after a pre-zero
+    of those rows.  y starts poisoned, so every row must be written.  Integers
+    bit-exact (and equal to the carry-merged plans); floats within the bound.
+    Covers a hub row over many tiles between empty rows, partition knobs,
+    4096-entry COO tiles and several right-hand sides."""
+    n = 9000
+    hub = _coo_rows(sp, 4000, n, [37] * n + [3999] * 5, list(range(n)) + [1, 2, 3, 4, 5], dt)
+    mats = [sp.gen_rmat(12, 60000, dtype=dt, seed=91), sp.gen_uniform(2001, 1999, 0, 30, dtype=dt, seed=92), hub,
+            _coo_rows(sp, 300, 50, [], [], dt), _coo_rows(sp, 5, 5, [4], [2], dt)]
+    for i, A in enumerate(mats):
+        x = sp.gen_vector(dt, A.ncols, seed=93 + i)
+        for kern in ("COO-row", "COO-nnz-rgrn", "COO-nnz"):
+            for kw in ({}, dict(nparts=16, nsub=4), dict(items_per_lane=16), dict(items_per_lane=8, nparts=3)):
+                y, plan = run(sp, ctx, kern, A, x, sp.opts(algo=5, **kw))
+                assert plan.info()["algo"] == 5
+                assert_parity(dt, y, A, x, f"mat{i}/{kern}/{kw}")
+                if dt <= 3:
+                    y0, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+                    assert np.array_equal(y, y0), f"mat{i}/{kern}/{kw}"
+        for kern in ("BCOO-block", "BCOO-nnz"):
+            for r, c in ((4, 4), (2, 2), (8, 8)):
+                for kw in ({}, dict(nparts=8, nsub=4)):
+                    y, plan = run(sp, ctx, kern, A, x, sp.opts(algo=5, row_block_size=r, col_block_size=c, **kw))
+                    assert plan.info()["algo"] == 5
+                    assert_parity(dt, y, A, x, f"mat{i}/{kern}/{r}x{c}/{kw}")
+        if i == 0:
+            X = np.stack([sp.gen_vector(dt, A.ncols, seed=99 + q) for q in range(3)], axis=1)
+            for kern in ("COO-nnz", "BCOO-nnz"):
+                Y, _ = run(sp, ctx, kern, A, X, sp.opts(algo=5), k=3)
+                assert_parity_multi(dt, Y, A, X, f"spmm {kern}")
+
+
+def test_atomic_merges_rejected(sp, ctx):
+    """algo=5 exists for COO and BCOO ids only; unknown algorithms are argument errors."""
+    A = sp.gen_rmat(10, 5000, dtype=sp.F32, seed=3)
+    for kern in ("CSR-nnz", "BCSR-nnz", "DCOO", "RBDBCOO-nnz"):
+        with pytest.raises(sp._lib.SparsePError) as e:
+            sp.Plan(ctx, kern, A, sp.opts(algo=5))
+        assert e.value.rc == -5, kern
+    with pytest.raises(sp._lib.SparsePError) as e:
+        sp.Plan(ctx, "COO-nnz", A, sp.opts(algo=6))
+    assert e.value.rc == -1
