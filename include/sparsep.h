@@ -416,8 +416,10 @@ typedef enum sp_algo {
      * (spmv/1D/COO-nnz/Makefile:7-9: one or 32 hashed mutexes around
      * y[row] += partial, spmv/1D/COO-nnz/dpu/task.c:819-885; likewise
      * spmv/1D/BCOO-nnz/dpu/task.c).  Its LOCKFREE default is the carry
-     * merge of SEGMENTED / SEGMENTED_SORTED.  1D COO
-     * (ids 2-4) and BCOO (ids 7, 8); other ids return SP_ERR_UNSUPPORTED.
+     * merge of SEGMENTED / SEGMENTED_SORTED.  The COO and BCOO ids, 1D
+     * (2-4, 7, 8) and 2D (10, 12, 14, 17, 18, 20, 23, 24, without
+     * lds_slice); other ids return SP_ERR_UNSUPPORTED: their kernels never
+     * share a row between workgroups.
      * Integer results are exact; floating-point sums of a row spread over
      * three or more tiles can differ in the last bits between runs.  Rows
      * the atomics touch are zeroed by a small kernel before the main one.

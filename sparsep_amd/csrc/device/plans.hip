@@ -902,6 +902,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             if ((rc = sp_partition_tiled(T, horz, tk.by_nnz, split.data()))) break;
         }
         int kind;
+        const bool atom = p->algo == SP_ALGO_SEGMENTED_ATOMIC;  // COO/BCOO tiles (the id was checked)
         bool col_order = false;  // column-ordered tiles (CSR merge path, COO), as for 1D
         if (tk.format == SP_FMT_CSR) {
             if (p->algo == SP_ALGO_AUTO) {
@@ -917,14 +918,16 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
         } else if (tk.format == SP_FMT_COO) {
             kind = WK_COO_SEG;
             col_order = p->algo != SP_ALGO_SEGMENTED;  // as for 1D COO
-            p->algo = SP_ALGO_SEGMENTED;
+            p->algo = atom ? SP_ALGO_SEGMENTED_ATOMIC : SP_ALGO_SEGMENTED;
         } else if (tk.format == SP_FMT_BCSR) {
             kind = WK_BCSR_VEC;
             p->algo = SP_ALGO_VECTOR;
         } else {
             kind = WK_BCOO_SEG;
-            p->algo = SP_ALGO_SEGMENTED;
+            p->algo = atom ? SP_ALGO_SEGMENTED_ATOMIC : SP_ALGO_SEGMENTED;
         }
+        // atomic merges need no pre-zero list here: the partial buffers are zeroed first
+        if (atom && p->opts.lds_slice) { rc = SP_ERR_UNSUPPORTED; break; }
         // skewed rows and more than 8M entries: 4096-entry tiles, as for 1D CSR/COO
         if ((kind == WK_CSR_MERGE || kind == WK_COO_SEG) && col_order && p->opts.items_per_lane == 0 &&
             !p->opts.lds_slice && maxlen > 4.0 * A0->nnz / std::max(n, 1u) + 64 && A0->nnz > (8u << 20) &&
@@ -1096,7 +1099,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
         if (tk.format == SP_FMT_CSR && kind == WK_CSR_MERGE)
             p->algo = col_order ? SP_ALGO_SEGMENTED_SORTED : SP_ALGO_SEGMENTED;
         else if (tk.format == SP_FMT_COO)
-            p->algo = col_order ? SP_ALGO_SEGMENTED_SORTED : SP_ALGO_SEGMENTED;
+            p->algo = atom ? SP_ALGO_SEGMENTED_ATOMIC : col_order ? SP_ALGO_SEGMENTED_SORTED : SP_ALGO_SEGMENTED;
         // algorithmic bytes: the tile formats' arrays, x once and y once (the partial
         // vectors and their reduction are overhead, not algorithm)
         {
@@ -1114,6 +1117,10 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
         const bool carries = kind == WK_CSR_MERGE || kind == WK_COO_SEG || kind == WK_BCOO_SEG;
         rc = b.finish(kind, carries, kind == WK_BCOO_SEG ? R : 1);
         p->work.zero_all = (kind == WK_COO_SEG || kind == WK_BCOO_SEG);
+        if (atom) {
+            p->work.atomic = true;
+            p->launches = 1;
+        }
         p->nsub = p->work.nunits;
     } while (0);
     sp_free_tiled(T);

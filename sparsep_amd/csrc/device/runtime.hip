@@ -177,7 +177,12 @@ static sp_plan *new_plan(sp_ctx *ctx, int id, int dtype, const sp_opts *opts) {
 static int check_algo(int id, const sp_opts *opts) {
     const int algo = opts ? opts->algo : SP_ALGO_AUTO;
     if (algo < SP_ALGO_AUTO || algo > SP_ALGO_SEGMENTED_ATOMIC) return SP_ERR_ARG;
-    if (algo == SP_ALGO_SEGMENTED_ATOMIC && !is_coo(id) && id != SP_K_BCOO_BLOCK && id != SP_K_BCOO_NNZ)
+    // the variants whose reference kernels merge shared rows (COO and BCOO formats, 1D and 2D)
+    static const int atomic_ids[] = {SP_K_COO_ROW, SP_K_COO_NNZ_RGRN, SP_K_COO_NNZ, SP_K_BCOO_BLOCK, SP_K_BCOO_NNZ,
+                                     SP_K_DCOO, SP_K_DBCOO, SP_K_RBDCOO, SP_K_RBDBCOO_BLOCK, SP_K_RBDBCOO_NNZ,
+                                     SP_K_BDCOO, SP_K_BDBCOO_BLOCK, SP_K_BDBCOO_NNZ};
+    if (algo == SP_ALGO_SEGMENTED_ATOMIC &&
+        std::find(std::begin(atomic_ids), std::end(atomic_ids), id) == std::end(atomic_ids))
         return SP_ERR_UNSUPPORTED;
     return SP_OK;
 }

@@ -209,12 +209,16 @@ def test_atomic_merges(sp, ctx, dt):
 
 
 def test_atomic_merges_rejected(sp, ctx):
-    """algo=5 exists for COO and BCOO ids only; unknown algorithms are argument errors."""
+    """algo=5 exists for the COO and BCOO ids only (1D and 2D); unknown algorithms
+    are argument errors; 2D atomic plans do not stage x slices."""
     A = sp.gen_rmat(10, 5000, dtype=sp.F32, seed=3)
-    for kern in ("CSR-nnz", "BCSR-nnz", "DCOO", "RBDBCOO-nnz"):
+    for kern in ("CSR-nnz", "BCSR-nnz", "DCSR", "RBDBCSR-nnz", "BDBCSR-block"):
         with pytest.raises(sp._lib.SparsePError) as e:
             sp.Plan(ctx, kern, A, sp.opts(algo=5))
         assert e.value.rc == -5, kern
     with pytest.raises(sp._lib.SparsePError) as e:
         sp.Plan(ctx, "COO-nnz", A, sp.opts(algo=6))
     assert e.value.rc == -1
+    with pytest.raises(sp._lib.SparsePError) as e:
+        sp.Plan(ctx, "DCOO", A, sp.opts(algo=5, vert=4, lds_slice=4096))
+    assert e.value.rc == -5

@@ -107,3 +107,24 @@ def test_2d_column_order_same_bits(sp, ctx, dt):
             assert plan.info()["algo"] == 3, (kern, kw)
             assert_parity(dt, y1, A, x, f"{kern}/{kw}")
             assert np.array_equal(y0, y1), f"{kern}/{kw}"
+
+
+@pytest.mark.parametrize("dt", [0, 2, 4, 5])
+def test_2d_atomic_merges(sp, ctx, dt):
+    """algo=5 on the 2D COO/BCOO tilings: rows shared between a stripe's tiles
+    are added into the zeroed partial buffers with atomics.  Integers bit-exact,
+    floats within the bound; also three right-hand sides."""
+    mats = [sp.gen_rmat(12, 40000, dtype=dt, seed=15), sp.gen_uniform(2000, 3001, 0, 25, dtype=dt, seed=18)]
+    kerns = ["DCOO", "DBCOO", "RBDCOO", "RBDBCOO-block", "RBDBCOO-nnz", "BDCOO", "BDBCOO-block", "BDBCOO-nnz"]
+    for i, A in enumerate(mats):
+        x = sp.gen_vector(dt, A.ncols, seed=27 + i)
+        for kern in kerns:
+            for kw in (dict(vert=1), dict(vert=4, nparts=16), dict(vert=8, nparts=64, nsub=4)):
+                y, plan = run(sp, ctx, kern, A, x, sp.opts(algo=5, **kw))
+                assert plan.info()["algo"] == 5
+                assert_parity(dt, y, A, x, f"mat{i}/{kern}/{kw}")
+    A = mats[0]
+    X = np.stack([sp.gen_vector(dt, A.ncols, seed=40 + q) for q in range(3)], axis=1)
+    for kern in ("DCOO", "BDBCOO-nnz"):
+        Y, _ = run(sp, ctx, kern, A, X, sp.opts(algo=5, vert=4), k=3)
+        assert_parity_multi(dt, Y, A, X, f"spmm {kern}")
