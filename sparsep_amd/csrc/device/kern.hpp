@@ -181,7 +181,9 @@ __global__ __launch_bounds__(NT) void csr_merge_kernel(const Prob *__restrict__ 
     }
     [[maybe_unused]] uint32_t staged = NONE;
     for (uint32_t t = t0; t < t1; t += tstep) {
-    const uint32_t pid = unit_pid[t], ci = t + pid;
+    // one-problem launches pass no unit_pid table: the descriptor and tile coordinates
+    // are then loaded together instead of after the problem id
+    const uint32_t pid = unit_pid ? unit_pid[t] : 0u, ci = t + pid;
     const Prob P = probs[pid];
     const gptr<const T> __restrict__ x = gp(io.x(pid, P));
     const gptr<T> __restrict__ y = gp(io.y(pid, P));

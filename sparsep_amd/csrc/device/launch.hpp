@@ -62,31 +62,33 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         }
     }
     switch (w.kind) {
-    case WK_CSR_MERGE:
+    case WK_CSR_MERGE: {
+        const uint32_t *upid = w.nprob == 1 ? nullptr : w.d_unit_pid;  // one problem: pid 0
         if constexpr (K == 1) {
             if (w.slice) {
                 switch (w.ipt) {
-                case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, 1, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
-                case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, 1, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+                case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, 1, true>), grid, block, 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+                case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, 1, true>), grid, block, 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
                 default: return SP_ERR_ARG;
                 }
                 break;
             }
         }
         switch (w.ipt) {
-        case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
-        case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+        case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+        case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, K>), grid, block, 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
         case 16:
             if constexpr (K == 1) {
                 // 4096-item tiles as 512 threads x 8 items: the registers (and occupancy) of 8 items per lane
-                hipLaunchKernelGGL((csr_merge_kernel<T, 8, 1, false, 2 * TPB>), grid, dim3(2 * TPB), 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits);
+                hipLaunchKernelGGL((csr_merge_kernel<T, 8, 1, false, 2 * TPB>), grid, dim3(2 * TPB), 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits);
             } else {
-                hipLaunchKernelGGL((csr_merge_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits);
+                hipLaunchKernelGGL((csr_merge_kernel<T, 16, K>), grid, block, 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits);
             }
             break;
         default: return SP_ERR_ARG;
         }
         break;
+    }
     case WK_CSR_VEC:
 #define SPD_VEC(Lv) case Lv: hipLaunchKernelGGL((csr_vector_kernel<T, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
         switch (w.lanes) {
