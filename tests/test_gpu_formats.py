@@ -222,3 +222,24 @@ def test_atomic_merges_rejected(sp, ctx):
     with pytest.raises(sp._lib.SparsePError) as e:
         sp.Plan(ctx, "DCOO", A, sp.opts(algo=5, vert=4, lds_slice=4096))
     assert e.value.rc == -5
+
+
+@pytest.mark.parametrize("dt", [2, 4])
+def test_coo_strided_output(sp, ctx, dt):
+    """COO SpMM into Y with a row stride wider than k: every row's first k
+    columns are written (empty rows zeroed by the fix-up) and the columns
+    past k keep what the caller had there."""
+    A = sp.gen_rmat(12, 30000, dtype=dt, seed=61)
+    k, ld = 3, 5
+    X = np.stack([sp.gen_vector(dt, A.ncols, seed=62 + q) for q in range(k)], axis=1)
+    for kern in ("COO-row", "COO-nnz"):
+        plan = sp.Plan(ctx, kern, A)
+        xd = sp.DeviceArray.from_numpy(ctx, np.ascontiguousarray(X))
+        yd = sp.DeviceArray(ctx, (A.nrows, ld), X.dtype)
+        yd.fill_bytes(0x5A)
+        plan.spmm(k, xd.addr, k, yd.addr, ld)
+        ctx.sync()
+        Y = yd.numpy()
+        assert_parity_multi(dt, np.ascontiguousarray(Y[:, :k]), A, X, kern)
+        pad = np.ascontiguousarray(Y[:, k:]).view(np.uint8)
+        assert np.array_equal(pad, np.full(pad.shape, 0x5A, np.uint8)), kern
