@@ -189,6 +189,10 @@ typedef struct sp_dist { int32_t kind; double lo, hi; } sp_dist;
 /* Fill `n` values of dtype from the distribution with the given seed. */
 int sp_gen_vector(int dtype, uint64_t n, sp_dist d, uint64_t seed, void *out);
 
+/* Reference: init_vector (spmv/1D/CSR-row/host/app.c:57-61): the compat
+ * input vector x[i] = i % 4 + 1. */
+int sp_init_vector(int dtype, uint64_t n, void *out);
+
 /* 2D 5-point stencil on a gx*gy grid: row i = y*gx + x; diagonal `diag`,
  * the (up to 4) neighbours drawn from `offdiag`.  Sorted, unique. */
 int sp_gen_stencil5(uint32_t gx, uint32_t gy, int dtype, sp_dist offdiag,
@@ -418,8 +422,12 @@ typedef enum sp_algo {
      * spmv/1D/BCOO-nnz/dpu/task.c).  Its LOCKFREE default is the carry
      * merge of SEGMENTED / SEGMENTED_SORTED.  The COO and BCOO ids, 1D
      * (2-4, 7, 8) and 2D (10, 12, 14, 17, 18, 20, 23, 24, without
-     * lds_slice); other ids return SP_ERR_UNSUPPORTED: their kernels never
-     * share a row between workgroups.
+     * lds_slice).  The BCSR ids (5, 6, 11, 15, 16, 21, 22) accept it and
+     * run their default plan: the reference's BCSR lock builds
+     * (spmv/1D/BCSR-nnz/Makefile:9) guard the 8-byte MRAM write granules
+     * that neighbouring tasklets share, and no BCSR kernel here shares a
+     * block row between workgroups without a carry merge.  The CSR ids
+     * return SP_ERR_UNSUPPORTED (the reference has no lock build for them).
      * Integer results are exact; floating-point sums of a row spread over
      * three or more tiles can differ in the last bits between runs.  Rows
      * the atomics touch are zeroed by a small kernel before the main one.

@@ -106,6 +106,7 @@ SIGNATURES = {
     "sp_rng_uniform": (D, [P]),
     "sp_rng_normal": (D, [P]),
     "sp_gen_vector": (I, [I, U64, sp_dist, U64, P]),
+    "sp_init_vector": (I, [I, U64, P]),
     "sp_gen_stencil5": (I, [U, U, I, sp_dist, D, U64, ctypes.POINTER(ctypes.POINTER(sp_coo))]),
     "sp_gen_rmat": (I, [U, U64, D, D, D, I, sp_dist, U64, ctypes.POINTER(ctypes.POINTER(sp_coo))]),
     "sp_gen_stencil27": (I, [U, U, I, I, U64, ctypes.POINTER(ctypes.POINTER(sp_coo))]),

@@ -266,6 +266,13 @@ def gen_vector(dtype, n, d=None, seed=7):
     return out
 
 
+def init_vector(dtype, n):
+    """The reference's compat input vector x[i] = i % 4 + 1."""
+    out = np.empty(n, dtype=NP_DTYPE[dtype])
+    check(lib().sp_init_vector(dtype, n, out.ctypes.data_as(ctypes.c_void_p)), "sp_init_vector")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # partitioners
 # ---------------------------------------------------------------------------

@@ -65,9 +65,12 @@ struct Builder {
         }
         if (carries) {
             w.carry_k = (kind == WK_BCOO_SEG || kind == WK_BCSR_SEG) ? 1 : 8;
-            const size_t n = (size_t)std::max<uint32_t>(w.nunits, 1) * slots;
-            if ((rc = plan_alloc(p, n * 4, (void **)&w.d_carry_row))) return rc;
-            if ((rc = plan_alloc(p, n * w.carry_k * acc_size(p->dtype), &w.d_carry_val))) return rc;
+            // atomic merges write no carries (batches allocate their own)
+            if (p->algo != SP_ALGO_SEGMENTED_ATOMIC) {
+                const size_t n = (size_t)std::max<uint32_t>(w.nunits, 1) * slots;
+                if ((rc = plan_alloc(p, n * 4, (void **)&w.d_carry_row))) return rc;
+                if ((rc = plan_alloc(p, n * w.carry_k * acc_size(p->dtype), &w.d_carry_val))) return rc;
+            }
         }
         if (kind == WK_CSR_MERGE && carries && !w.nzero) {
             // tile t carries row c0[t + pid + 1] when that row exists; consecutive

@@ -160,6 +160,16 @@ static uint64_t csr_bytes(uint64_t n, uint64_t m, uint64_t nnz, uint64_t sv) {
     return nnz * (sv + 4) + (n + 1) * 4 + (m + n) * sv;
 }
 
+// The BCSR variants, 1D and 2D.  Their reference builds also take CGLOCK /
+// FGLOCK (spmv/1D/BCSR-nnz/Makefile:9), where the locks guard the 8-byte MRAM
+// write granules that neighbouring tasklets' block rows share.  No BCSR
+// kernel here shares a block row between workgroups without a carry merge,
+// so a lock build maps to the default plan.
+static bool is_block_csr_id(int id) {
+    return id == SP_K_BCSR_BLOCK || id == SP_K_BCSR_NNZ || id == SP_K_DBCSR || id == SP_K_RBDBCSR_BLOCK ||
+           id == SP_K_RBDBCSR_NNZ || id == SP_K_BDBCSR_BLOCK || id == SP_K_BDBCSR_NNZ;
+}
+
 static sp_plan *new_plan(sp_ctx *ctx, int id, int dtype, const sp_opts *opts) {
     sp_plan *p = new sp_plan();
     p->ctx = ctx;
@@ -169,6 +179,7 @@ static sp_plan *new_plan(sp_ctx *ctx, int id, int dtype, const sp_opts *opts) {
     else sp_opts_default(&p->opts);
     if (p->opts.row_block_size == 0) p->opts.row_block_size = 4;
     if (p->opts.col_block_size == 0) p->opts.col_block_size = 4;
+    if (p->opts.algo == SP_ALGO_SEGMENTED_ATOMIC && is_block_csr_id(id)) p->opts.algo = SP_ALGO_AUTO;
     p->algo = p->opts.algo;
     return p;
 }
@@ -177,11 +188,12 @@ static sp_plan *new_plan(sp_ctx *ctx, int id, int dtype, const sp_opts *opts) {
 static int check_algo(int id, const sp_opts *opts) {
     const int algo = opts ? opts->algo : SP_ALGO_AUTO;
     if (algo < SP_ALGO_AUTO || algo > SP_ALGO_SEGMENTED_ATOMIC) return SP_ERR_ARG;
-    // the variants whose reference kernels merge shared rows (COO and BCOO formats, 1D and 2D)
+    // the variants whose reference kernels merge shared rows under locks (COO and BCOO
+    // formats, 1D and 2D; BCSR ids take the request and run their default plan)
     static const int atomic_ids[] = {SP_K_COO_ROW, SP_K_COO_NNZ_RGRN, SP_K_COO_NNZ, SP_K_BCOO_BLOCK, SP_K_BCOO_NNZ,
                                      SP_K_DCOO, SP_K_DBCOO, SP_K_RBDCOO, SP_K_RBDBCOO_BLOCK, SP_K_RBDBCOO_NNZ,
                                      SP_K_BDCOO, SP_K_BDBCOO_BLOCK, SP_K_BDBCOO_NNZ};
-    if (algo == SP_ALGO_SEGMENTED_ATOMIC &&
+    if (algo == SP_ALGO_SEGMENTED_ATOMIC && !is_block_csr_id(id) &&
         std::find(std::begin(atomic_ids), std::end(atomic_ids), id) == std::end(atomic_ids))
         return SP_ERR_UNSUPPORTED;
     return SP_OK;

@@ -99,6 +99,14 @@ int sp_gen_vector(int dtype, uint64_t n, sp_dist d, uint64_t seed, void *out) {
     return SP_OK;
 }
 
+/* init_vector: x[i] = (val_dt)(i % 4 + 1), spmv/1D/CSR-row/host/app.c:57-61
+ * (the 2D copy at spmv/2D/DCSR/host/app.c:75 is the same). */
+int sp_init_vector(int dtype, uint64_t n, void *out) {
+    if (!sp_valid_dtype(dtype) || (!out && n)) return SP_ERR_ARG;
+    for (uint64_t i = 0; i < n; i++) sp_put(out, dtype, i, (double)(i % 4 + 1));
+    return SP_OK;
+}
+
 void sp_radix_sort_u64(uint64_t *keys, uint64_t n) {
     if (n < 2) return;
     if (n <= 256) { /* a 64K-bucket pass costs far more than insertion sort here */

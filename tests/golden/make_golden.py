@@ -254,6 +254,21 @@ def main():
             put({"kind": "tiled_bcoo", "tiling": tiling, "by_nnz": int(dirname.endswith("-nnz")), "nrows": nr,
                  "ncols": nc, "horz": horz, "vert": vert, "dims": [int(d) for d in dims]}, **arrays)
 
+    # COO-nnz inter-DPU split with straddling rows (spmv/1D/COO-nnz/host/app.c:118-205)
+    L = ctypes.CDLL(os.path.join(orc.REFDIR, "libref_coo_split.so"))
+    for nrows, Pn in ((40, 7), (300, 16), (17, 200), (1000, 64)):
+        for kind in ("uniform", "skew", "empty"):
+            rows = row_lengths(rng, nrows, kind).astype(np.uint32)
+            rows[0] += 1
+            nnz = int(rows.sum())
+            outs = [np.zeros(Pn, np.uint32) for _ in range(5)]
+            tp = min(Pn, 8)
+            ts, tc = np.zeros(tp * 16, np.uint32), np.zeros(tp * 16, np.uint32)
+            L.ref_coo_nnz_split(U(nrows), U(nnz), P(rows), U(Pn), *[P(o) for o in outs], U(tp), P(ts), P(tc))
+            put({"kind": "coo_split", "P": Pn, "nnz": nnz, "tsk_parts": tp}, rows=rows,
+                **dict(zip(["prev_rows", "rows_per_part", "prev_nnz", "nnz_per_part", "merge"], outs)),
+                tstart=ts, tcount=tc)
+
     np.savez_compressed(os.path.join(HERE, "host_surface.npz"), **ARR)
     with open(os.path.join(HERE, "host_surface.json"), "w") as f:
         json.dump(CASES, f, indent=0)

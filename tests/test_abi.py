@@ -2,6 +2,7 @@
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,3 +57,11 @@ def test_c_caller_of_the_host_surface(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert out.stdout.startswith("ok")
+
+
+def test_init_vector_compat(sp):
+    """init_vector (spmv/1D/CSR-row/host/app.c:57-61): x[i] = i % 4 + 1 in val_dt."""
+    for dt in range(6):
+        x = sp.init_vector(dt, 1001)
+        assert x.dtype == sp.NP_DTYPE[dt]
+        assert np.array_equal(x, (np.arange(1001) % 4 + 1).astype(sp.NP_DTYPE[dt]))

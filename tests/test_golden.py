@@ -34,7 +34,7 @@ def ids(kind):
 def test_fixture_covers_every_kind():
     kinds = {c["kind"] for c in CASES}
     assert kinds == {"coo2csr", "sort_coo", "bcsr", "bcoo", "part1d", "tsklt", "part_bcsr", "part_bcoo", "mtx",
-                     "tiled_csr", "tiled_coo", "tiled_bcoo"}
+                     "tiled_csr", "tiled_coo", "tiled_bcoo", "coo_split"}
 
 
 @pytest.mark.parametrize("c", cases("coo2csr"), ids=ids("coo2csr"))
@@ -165,3 +165,15 @@ def test_tiled_bcoo(sp, c):
     if c["tiling"]:
         split = sp.partition_tiled(T, c["horz"], c["by_nnz"]).reshape(c["vert"], c["horz"] + 1)
         assert np.array_equal(np.diff(split.astype(np.int64), axis=1).ravel(), a["blocks"])
+
+
+@pytest.mark.parametrize("c", cases("coo_split"), ids=ids("coo_split"))
+def test_coo_nnz_split(sp, c):
+    a = arrays(c)
+    s = sp.partition_coo_nnz(a["rows"], c["nnz"], c["P"])
+    for name in ("prev_rows", "rows_per_part", "prev_nnz", "nnz_per_part", "merge"):
+        assert np.array_equal(s[name], a[name]), name
+    for i in range(c["tsk_parts"]):
+        start, cnt = sp.partition_even(int(s["nnz_per_part"][i]), 16)
+        assert np.array_equal(start, a["tstart"][i * 16:(i + 1) * 16])
+        assert np.array_equal(cnt, a["tcount"][i * 16:(i + 1) * 16])

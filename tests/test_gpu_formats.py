@@ -209,13 +209,23 @@ def test_atomic_merges(sp, ctx, dt):
 
 
 def test_atomic_merges_rejected(sp, ctx):
-    """algo=5 exists for the COO and BCOO ids only (1D and 2D); unknown algorithms
-    are argument errors; 2D atomic plans do not stage x slices."""
+    """algo=5 exists for the COO and BCOO ids (1D and 2D); the BCSR ids take it
+    as their default plan (no block row is shared between workgroups); the CSR
+    ids refuse it; unknown algorithms are argument errors; 2D atomic plans do
+    not stage x slices."""
     A = sp.gen_rmat(10, 5000, dtype=sp.F32, seed=3)
-    for kern in ("CSR-nnz", "BCSR-nnz", "DCSR", "RBDBCSR-nnz", "BDBCSR-block"):
+    for kern in ("CSR-row", "CSR-nnz", "DCSR", "RBDCSR", "BDCSR"):
         with pytest.raises(sp._lib.SparsePError) as e:
             sp.Plan(ctx, kern, A, sp.opts(algo=5))
         assert e.value.rc == -5, kern
+    A2 = sp.gen_rmat(10, 5000, dtype=sp.I32, seed=3)
+    x = sp.gen_vector(sp.I32, A2.ncols, seed=4)
+    for kern in ("BCSR-block", "BCSR-nnz", "DBCSR", "RBDBCSR-block", "RBDBCSR-nnz", "BDBCSR-block", "BDBCSR-nnz"):
+        y5, p5 = run(sp, ctx, kern, A2, x, sp.opts(algo=5))
+        y0, p0 = run(sp, ctx, kern, A2, x, sp.opts())
+        assert p5.info()["algo"] == p0.info()["algo"], kern
+        assert np.array_equal(y5, y0), kern
+        assert_parity(sp.I32, y5, A2, x, kern)
     with pytest.raises(sp._lib.SparsePError) as e:
         sp.Plan(ctx, "COO-nnz", A, sp.opts(algo=6))
     assert e.value.rc == -1

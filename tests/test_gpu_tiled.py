@@ -109,10 +109,12 @@ def test_2d_column_order_same_bits(sp, ctx, dt):
             assert np.array_equal(y0, y1), f"{kern}/{kw}"
 
 
-@pytest.mark.parametrize("dt", [0, 2, 4, 5])
+@pytest.mark.parametrize("dt", [0, 1, 2, 3, 4, 5])
 def test_2d_atomic_merges(sp, ctx, dt):
     """algo=5 on the 2D COO/BCOO tilings: rows shared between a stripe's tiles
-    are added into the zeroed partial buffers with atomics.  Integers bit-exact,
+    are added into the zeroed partial buffers with atomics (the 1- and 2-byte
+    integers by compare-and-swap on the enclosing word, 8-byte integers by a
+    64-bit add).  Integers bit-exact and equal to the carry-merged plans,
     floats within the bound; also three right-hand sides."""
     mats = [sp.gen_rmat(12, 40000, dtype=dt, seed=15), sp.gen_uniform(2000, 3001, 0, 25, dtype=dt, seed=18)]
     kerns = ["DCOO", "DBCOO", "RBDCOO", "RBDBCOO-block", "RBDBCOO-nnz", "BDCOO", "BDBCOO-block", "BDBCOO-nnz"]
@@ -123,6 +125,9 @@ def test_2d_atomic_merges(sp, ctx, dt):
                 y, plan = run(sp, ctx, kern, A, x, sp.opts(algo=5, **kw))
                 assert plan.info()["algo"] == 5
                 assert_parity(dt, y, A, x, f"mat{i}/{kern}/{kw}")
+                if dt <= 3:
+                    y0, _ = run(sp, ctx, kern, A, x, sp.opts(**kw))
+                    assert np.array_equal(y, y0), f"mat{i}/{kern}/{kw}"
     A = mats[0]
     X = np.stack([sp.gen_vector(dt, A.ncols, seed=40 + q) for q in range(3)], axis=1)
     for kern in ("DCOO", "BDBCOO-nnz"):
