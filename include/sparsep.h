@@ -42,7 +42,9 @@ typedef enum sp_status {
     SP_ERR_HIP = -4,          /* HIP runtime error                         */
     SP_ERR_UNSUPPORTED = -5,  /* variant/dtype/shape not supported         */
     SP_ERR_NODEVICE = -6,     /* no HIP device                             */
-    SP_ERR_STATE = -7         /* object in the wrong state                 */
+    SP_ERR_STATE = -7,        /* object in the wrong state                 */
+    SP_ERR_RANGE = -8         /* input larger than a fixed-capacity entry  */
+                              /* point holds                               */
 } sp_status;
 
 const char *sp_strerror(int rc);
@@ -511,6 +513,22 @@ int sp_spmv(sp_plan *plan, const void *x, void *y);
 /* Y = A X for k right-hand sides; X is ncols x k row-major with leading
  * dimension ldx >= k, Y is nrows x k with ldy >= k (device pointers). */
 int sp_spmm(sp_plan *plan, int k, const void *X, int ldx, void *Y, int ldy);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): y = A x for a CSR matrix whose rowptr[nrows+1],
+ * colind[nnz] and values[nnz] are already in device memory, with no plan.
+ * It runs the entry-balanced merge-path kernel of a CSR-nnz plan over
+ * row-ordered 2048-item tiles and allocates nothing: tile coordinates and
+ * cross-tile carries live in a library-owned static device buffer of about
+ * 1.6 MB per device, held until the process ends.  Limit: nrows + nnz <=
+ * SP_NOALLOC_MAX_ITEMS, else SP_ERR_RANGE.  Stream-ordered on the context
+ * stream.  Every call on one GPU shares that buffer, so calls that overlap
+ * on one GPU (other streams or contexts) race. */
+#define SP_NOALLOC_MAX_ITEMS 134217728u
+int sp_spmv_csr_noalloc(sp_ctx *ctx, int dtype, uint32_t nrows, uint32_t ncols, uint32_t nnz,
+                        const uint32_t *rowptr, const uint32_t *colind, const void *values,
+                        const void *x, void *y);
 
 /* Time `iters` SpMVs with HIP events recorded on the ctx stream around each
  * launch group.  flush_bytes > 0 overwrites a scratch buffer of that size

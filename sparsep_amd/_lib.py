@@ -159,6 +159,7 @@ SIGNATURES = {
     "sp_plan_get_info": (I, [P, ctypes.POINTER(sp_plan_info)]),
     "sp_spmv": (I, [P, P, P]),
     "sp_spmm": (I, [P, I, P, I, P, I]),
+    "sp_spmv_csr_noalloc": (I, [P, I, U, U, U, P, P, P, P, P]),
     "sp_time_spmv": (I, [P, P, P, I, U64, ctypes.POINTER(ctypes.c_float)]),
     "sp_time_spmm": (I, [P, I, P, I, P, I, I, U64, ctypes.POINTER(ctypes.c_float)]),
     "sp_time_spmv_split": (I, [P, P, P, I, U64, ctypes.POINTER(ctypes.c_float),

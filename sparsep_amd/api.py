@@ -405,6 +405,20 @@ def gpu_present():
     return os.path.exists("/dev/kfd")
 
 
+def spmv_csr_noalloc(ctx, dtype, nrows, ncols, nnz, rowptr_ptr, colind_ptr, values_ptr, x_ptr, y_ptr):
+    """sp_spmv_csr_noalloc: y = A x over CSR arrays already in device memory,
+    without a plan and without allocating (fixed scratch, nrows + nnz <=
+    NOALLOC_MAX_ITEMS).  Returns the status code instead of raising, so
+    callers can check SP_ERR_RANGE."""
+    return lib().sp_spmv_csr_noalloc(ctx.ptr, dtype, nrows, ncols, nnz, ctypes.c_void_p(rowptr_ptr),
+                                     ctypes.c_void_p(colind_ptr), ctypes.c_void_p(values_ptr),
+                                     ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr))
+
+
+NOALLOC_MAX_ITEMS = 134217728
+ERR_RANGE = -8
+
+
 class DeviceArray:
     """Device buffer owned through the C ABI (sp_dmalloc / sp_dfree) with
     synchronous host copies on the context stream."""

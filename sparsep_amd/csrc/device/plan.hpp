@@ -61,6 +61,10 @@ struct sp_plan {
 int plan_alloc(sp_plan *p, size_t bytes, void **out);
 int plan_upload(sp_plan *p, const void *host, size_t bytes, void **out);
 
+// Allocation-free entry point (noalloc.hip): look up this device's static
+// scratch buffer; the device must be current.
+int sp_noalloc_setup(int dev);
+
 // Plan builders (plans.hip).
 int build_csr_plan(sp_plan *p, const sp_csr *A);
 int build_coo_plan(sp_plan *p, const sp_coo *A);

@@ -49,6 +49,9 @@ extern "C" int sp_ctx_create(int dev, sp_ctx **out) {
     strncpy(c->arch, prop.gcnArchName, sizeof(c->arch) - 1);
     if (hipStreamCreate(&c->own_stream) != hipSuccess) { delete c; return SP_ERR_HIP; }
     c->stream = c->own_stream;
+    // the allocation-free entry point's scratch address on this device (a
+    // failure here is retried by that entry point's first call)
+    (void)sp_noalloc_setup(dev);
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev_mid) != hipSuccess) {
         hipStreamDestroy(c->own_stream);

@@ -13,6 +13,7 @@ const char *sp_strerror(int rc) {
     case SP_ERR_UNSUPPORTED: return "unsupported variant, dtype or shape";
     case SP_ERR_NODEVICE: return "no HIP device";
     case SP_ERR_STATE: return "object in wrong state";
+    case SP_ERR_RANGE: return "input larger than the entry point's fixed capacity";
     default: return "unknown error";
     }
 }
