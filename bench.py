@@ -73,6 +73,60 @@ def parity_ok(orc, dt, A, x, y):
     return bool(np.all(np.abs(y.astype(np.float64) - yref) <= tol))
 
 
+# The reference CPU path of each kernel id: which spmv_host restatement it is
+# timed with (spmv/1D/CSR-row/host/app.c:66-77, COO-row:70-74, BCSR-block:70-85,
+# BCOO-block:72-86, the 2D tile-major copies such as spmv/2D/DCSR/host/app.c:84-103)
+# and, for the 2D ids, the OpenMP vertical merge (spmv/2D/DCSR/host/app.c:340-351).
+_CPU_SRC = {0: "spmv/1D/CSR-row/host/app.c:66-77", 1: "spmv/1D/COO-row/host/app.c:70-74",
+            2: "spmv/1D/BCSR-block/host/app.c:70-85", 3: "spmv/1D/BCOO-block/host/app.c:72-86"}
+_TILED = {9: (0, 0), 10: (0, 1), 11: (0, 2), 12: (0, 3), 13: (1, 0), 14: (1, 1), 15: (1, 2), 16: (1, 2),
+          17: (1, 3), 18: (1, 3), 19: (2, 0), 20: (2, 1), 21: (2, 2), 22: (2, 2), 23: (2, 3), 24: (2, 3)}
+
+
+def cpu_baseline_for(sp, orc, kid, A, x, info, budget_s):
+    """Times the reference CPU path of kernel id `kid` on the same matrix and x:
+    1 thread, the restated spmv_host of its format; 2D ids also report the host
+    merge of the partial vectors on all cores.  Bounded by budget_s seconds."""
+    dt = A.dtype
+    r, c = info["row_block_size"] or 4, info["col_block_size"] or 4
+    merge = None
+    if kid <= 1:
+        C = A.to_csr()
+        ms, done, _ = orc.time_csr(dt, C.rowptr, C.colind, C.values, x, iters=1000, warmup=1, omp_threads=0,
+                                   budget_s=budget_s)
+        src = _CPU_SRC[0]
+    elif kid <= 4:
+        ms, done, _ = orc.time_coo(dt, A.nrows, A.rowind, A.colind, A.values, x, warmup=1, budget_s=budget_s)
+        src = _CPU_SRC[1]
+    elif kid <= 8:
+        B = A.to_csr().to_bcsr(r, c)
+        if kid <= 6:
+            ms, done, _ = orc.time_bcsr(dt, A.nrows, A.ncols, r, c, B.browptr, B.bcolind, B.bval, x, warmup=1,
+                                        budget_s=budget_s)
+            src = _CPU_SRC[2]
+        else:
+            O = B.to_bcoo()
+            ms, done, _ = orc.time_bcoo(dt, A.nrows, A.ncols, r, c, O.browind, O.bcolind, O.bval, x, warmup=1,
+                                        budget_s=budget_s)
+            src = _CPU_SRC[3]
+    else:
+        tiling, fmt = _TILED[kid]
+        vert = max(1, info["vert"])
+        horz = max(1, info["nparts"] // vert)
+        T = sp.coo2tiled(A, tiling, fmt, horz, vert, r, c)
+        ms, done, _ = orc.time_tiled(dt, T, A.nrows, A.ncols, x, warmup=1, budget_s=budget_s)
+        src = f"the 2D tile-major spmv_host of spmv/2D/{sp.KERNELS[kid]}/host/app.c"
+        mms, mdone = orc.time_merge(dt, vert, A.nrows, orc.num_cpus(), warmup=1, budget_s=min(2.0, budget_s))
+        merge = {"ms": round(mms, 3), "threads": orc.num_cpus(), "parts": vert,
+                 "source": "spmv/2D/DCSR/host/app.c:340-351 (OpenMP vertical merge)"}
+    cpu = {"value": round(info["algo_bytes"] / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+           "sample": f"full matrix, median of {done} runs of the C restatement of {src}, 1 thread, {ms:.1f} ms/run",
+           "host_cpus": orc.num_cpus()}
+    if merge:
+        cpu["host_merge"] = merge
+    return cpu
+
+
 # --------------------------------------------------------------------------
 def wl_rmat(sp, ctx, args, orc):
     import numpy as np
@@ -212,13 +266,15 @@ def wl_formats(sp, ctx, args, orc):
             mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
             ok = parity_ok(orc, dt, A, x, yd.numpy())
             gbs = info["algo_bytes"] / (mean_dom * 1e-3) / 1e9
-            log(f"[bench] formats {DT_NAME[dt]} {kname:13s} {mean_ms * 1e3:8.2f} us  {gbs:7.0f} GB/s  parity {ok}")
+            cpu = None if args.no_cpu else cpu_baseline_for(sp, orc, sp.KID[kbase], A, x, info, args.cpu_budget_each)
+            log(f"[bench] formats {DT_NAME[dt]} {kname:13s} {mean_ms * 1e3:8.2f} us  {gbs:7.0f} GB/s  parity {ok}"
+                + (f"  cpu {cpu['value']:.2f} GB/s" if cpu else ""))
             cfg = {"workload": f"R-MAT 2^20, 2^24 nnz, {kname}, MALL flushed", "kernel": kbase, "algo": info["algo"],
                    "nnz": A.nnz, "algo_bytes": info["algo_bytes"]}
             out.append(line(args, info["algo_bytes"] / (mean_ms * 1e-3) / 1e9, mean_ms, DT_NAME[dt], cfg,
                             roofline={"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                                       "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
-                            extra={"parity_vs_oracle": ok, "launches": info["launches"]}))
+                            cpu=cpu, extra={"parity_vs_oracle": ok, "launches": info["launches"]}))
             del plan
     return out
 
@@ -244,6 +300,7 @@ def wl_stencil27(sp, ctx, args, orc):
             mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
             ok = parity_ok(orc, dt, A, x, yd.numpy())
             gbs = info["algo_bytes"] / (mean_dom * 1e-3) / 1e9
+            cpu = None if args.no_cpu else cpu_baseline_for(sp, orc, sp.KID[kname], A, x, info, args.cpu_budget_each)
             tag = f"{kname} {be}x{be}" if "B" in kname.replace("BD", "") else kname
             log(f"[bench] stencil27 {DT_NAME[dt]} {tag:17s} {mean_ms * 1e3:8.2f} us {gbs:7.0f} GB/s "
                 f"(dominant) parity {ok} build {build:.1f}s")
@@ -252,6 +309,7 @@ def wl_stencil27(sp, ctx, args, orc):
             out.append(line(args, info["algo_bytes"] / (mean_ms * 1e-3) / 1e9, mean_ms, DT_NAME[dt], cfg,
                             roofline={"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                                       "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                            cpu=cpu,
                             extra={"parity_vs_oracle": ok, "launches": info["launches"], "build_s": round(build, 1)}))
             del plan
     return out
@@ -352,6 +410,7 @@ def wl_all25(sp, ctx, args, orc):
         build = time.time() - t0
         mean_ms, mean_dom = measure(sp, plan, xd, yd, args, args.flush_mb << 20)
         ok = parity_ok(orc, dt, A, x, yd.numpy())
+        cpu = None if args.no_cpu else cpu_baseline_for(sp, orc, kid, A, x, info, args.cpu_budget_each)
         if kid == 0:
             csr_bytes = info["algo_bytes"]
         gbs = info["algo_bytes"] / (mean_ms * 1e-3) / 1e9
@@ -364,6 +423,7 @@ def wl_all25(sp, ctx, args, orc):
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": round(info["algo_bytes"] / (mean_dom * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                                   "traffic": None},
+                        cpu=cpu,
                         extra={"parity_vs_oracle": ok, "launches": info["launches"], "build_s": round(build, 1),
                                "csr_equivalent_gbs": round(csr_bytes / (mean_ms * 1e-3) / 1e9, 1)}))
         del plan
@@ -547,6 +607,8 @@ def main():
     ap.add_argument("--workload", default="rmat", choices=sorted(WORKLOADS))
     ap.add_argument("--flush-mb", type=int, default=1024)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline timing")
+    ap.add_argument("--cpu-budget-each", type=float, default=3.0,
+                    help="formats / stencil27 / all25: seconds of CPU baseline timing per variant")
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--nnz-log2", type=int, default=24)
     ap.add_argument("--vert", type=int, default=0, help="stencil27: vertical partitions of the 2D kernels")

@@ -17,6 +17,8 @@
  *                  BDBCSR-block:92-114 and the other 2D spmv_host copies
  *   orc_merge_*    spmv/1D/COO-nnz/host/app.c:275-289, spmv/2D/DCSR/host/app.c:340-351
  *
+ * The orc_time_* functions time these restatements for the CPU baseline.
+ *
  * plus an independent high-precision reference (orc_ref_csr: long double
  * accumulation and the per-row sum of |a_ij x_j| for error bounds) and an
  * OpenMP row-parallel CSR used only as a labelled "not reference" CPU
@@ -325,6 +327,100 @@ double orc_time_csr(int dt, uint32_t n, const uint32_t *rp, const uint32_t *ci, 
     if (done == 0) return -1.0;
     qsort(t, (size_t)done, sizeof(double), cmp_d);
     return t[done / 2];
+}
+
+/* Times the other spmv_host restatements (single thread, as the
+ * reference) and the 2D vertical merge (OpenMP over `nthreads`, the
+ * reference's -n flag): median ms of `iters` runs after `warmup`, stopping
+ * once `budget_s` seconds are spent.  Each run starts from the y the
+ * restatement itself zeroes (the reference's calloc). */
+typedef struct {
+    int kind, dt, fmt;
+    uint32_t nrows, ncols, n2, R, C;
+    uint64_t nb;
+    const uint32_t *a0, *a1, *a2, *a3, *a4, *a5, *a6, *a7, *a8;
+    const uint64_t *p0, *p1;
+    const void *v, *x;
+    void *y;
+    int nthreads;
+} job_t;
+
+static void run_job(const job_t *j) {
+    switch (j->kind) {
+    case 0: orc_spmv_coo(j->dt, j->nrows, j->n2, j->a0, j->a1, j->v, j->x, j->y); break;
+    case 1: orc_spmv_bcsr(j->dt, j->nrows, j->ncols, j->R, j->C, j->n2, j->a0, j->a1, j->v, j->x, j->y); break;
+    case 2: orc_spmv_bcoo(j->dt, j->nrows, j->ncols, j->R, j->C, j->nb, j->a0, j->a1, j->v, j->x, j->y); break;
+    case 3:
+        orc_spmv_tiled(j->dt, j->fmt, j->nrows, j->ncols, j->n2, j->a0, j->a1, j->a2, j->a3, j->p0, j->p1, j->a4,
+                       j->a5, j->a6, j->R, j->C, j->v, j->x, j->y);
+        break;
+    default: orc_merge_parts(j->dt, j->n2, j->nb, (uint64_t)j->ncols, j->y, j->nthreads); break;
+    }
+}
+
+static double time_job(const job_t *j, int iters, int warmup, double budget_s, int *iters_done) {
+    double t[1024];
+    if (iters > 1024) iters = 1024;
+    int done = 0;
+    const double start = now_s();
+    for (int it = -warmup; it < iters; it++) {
+        double t0 = now_s();
+        run_job(j);
+        double t1 = now_s();
+        if (it >= 0) t[done++] = (t1 - t0) * 1e3;
+        if (budget_s > 0 && t1 - start > budget_s && done >= 1) break;
+    }
+    if (iters_done) *iters_done = done;
+    if (done == 0) return -1.0;
+    qsort(t, (size_t)done, sizeof(double), cmp_d);
+    return t[done / 2];
+}
+
+double orc_time_coo(int dt, uint32_t n, uint32_t nnz, const uint32_t *ri, const uint32_t *ci, const void *v,
+                    const void *x, void *y, int iters, int warmup, double budget_s, int *iters_done) {
+    job_t j = {0};
+    j.kind = 0, j.dt = dt, j.nrows = n, j.n2 = nnz, j.a0 = ri, j.a1 = ci, j.v = v, j.x = x, j.y = y;
+    return time_job(&j, iters, warmup, budget_s, iters_done);
+}
+
+double orc_time_bcsr(int dt, uint32_t nrows, uint32_t ncols, uint32_t R, uint32_t C, uint32_t nbr,
+                     const uint32_t *bp, const uint32_t *bc, const void *bv, const void *x, void *y, int iters,
+                     int warmup, double budget_s, int *iters_done) {
+    job_t j = {0};
+    j.kind = 1, j.dt = dt, j.nrows = nrows, j.ncols = ncols, j.R = R, j.C = C, j.n2 = nbr, j.a0 = bp, j.a1 = bc;
+    j.v = bv, j.x = x, j.y = y;
+    return time_job(&j, iters, warmup, budget_s, iters_done);
+}
+
+double orc_time_bcoo(int dt, uint32_t nrows, uint32_t ncols, uint32_t R, uint32_t C, uint64_t nb,
+                     const uint32_t *br, const uint32_t *bc, const void *bv, const void *x, void *y, int iters,
+                     int warmup, double budget_s, int *iters_done) {
+    job_t j = {0};
+    j.kind = 2, j.dt = dt, j.nrows = nrows, j.ncols = ncols, j.R = R, j.C = C, j.nb = nb, j.a0 = br, j.a1 = bc;
+    j.v = bv, j.x = x, j.y = y;
+    return time_job(&j, iters, warmup, budget_s, iters_done);
+}
+
+double orc_time_tiled(int dt, int fmt, uint32_t nrows, uint32_t ncols, uint32_t ntiles, const uint32_t *row_off,
+                      const uint32_t *rows, const uint32_t *col_off, const uint32_t *col_w, const uint64_t *ptr_off,
+                      const uint64_t *ent_off, const uint32_t *ptr, const uint32_t *ri, const uint32_t *ci,
+                      uint32_t R, uint32_t C, const void *v, const void *x, void *y, int iters, int warmup,
+                      double budget_s, int *iters_done) {
+    job_t j = {0};
+    j.kind = 3, j.dt = dt, j.fmt = fmt, j.nrows = nrows, j.ncols = ncols, j.n2 = ntiles;
+    j.a0 = row_off, j.a1 = rows, j.a2 = col_off, j.a3 = col_w, j.p0 = ptr_off, j.p1 = ent_off;
+    j.a4 = ptr, j.a5 = ri, j.a6 = ci, j.R = R, j.C = C, j.v = v, j.x = x, j.y = y;
+    return time_job(&j, iters, warmup, budget_s, iters_done);
+}
+
+/* parts: nparts vectors of n values at `stride`; merged into the first. */
+double orc_time_merge(int dt, uint32_t nparts, uint64_t n, uint64_t stride, void *parts, int nthreads, int iters,
+                      int warmup, double budget_s, int *iters_done) {
+    job_t j = {0};
+    j.kind = 4, j.dt = dt, j.n2 = nparts, j.nb = n, j.ncols = (uint32_t)stride, j.y = parts;
+    j.nthreads = nthreads < 1 ? 1 : nthreads;
+    if (stride > UINT32_MAX) return -1.0;
+    return time_job(&j, iters, warmup, budget_s, iters_done);
 }
 
 int orc_num_cpus(void) {

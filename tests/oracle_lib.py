@@ -112,6 +112,92 @@ def time_csr(dt, rowptr, colind, vals, x, iters, warmup, omp_threads=0, budget_s
     return ms, done.value, y
 
 
+def tile_args(T, nrows):
+    """The tile descriptors orc_spmv_tiled takes, read from a built tiling."""
+    nt, vert = T.ntiles, T.vert_partitions
+    cuts = T.vert_tile_widths.astype(np.int64)
+    t = np.arange(nt)
+    c = t % vert
+    if T.tiling == 0:
+        row_off = (t // vert) * T.tile_height
+    else:
+        row_off = np.zeros(nt, np.int64)
+    rows = np.full(nt, T.tile_height if T.tiling == 0 else nrows, np.int64)
+    col_off, col_w = cuts[c], cuts[c + 1] - cuts[c]
+    if T.format in (0, 1):
+        ptr_off = t * (T.tile_height + 1)
+        ent_off, ci, vals = T.nnz_offset, T.dcolind, T.dval
+        ptr = T.drowptr if T.format == 0 else None
+        ri = T.drowind if T.format == 1 else None
+        R = C = 1
+    else:
+        ptr_off = t * (T.num_block_rows + 1)
+        ent_off, ci, vals = T.block_offset, T.bcolind, T.bval
+        ptr = T.browptr if T.format == 2 else None
+        ri = T.browind if T.format == 3 else None
+        R, C = T.row_block_size, T.col_block_size
+    return dict(row_off=row_off, rows=rows, col_off=col_off, col_w=col_w, ptr_off=ptr_off, ent_off=ent_off,
+                ptr=ptr, ri=ri, ci=ci, R=R, C=C, vals=vals)
+
+
+def _timed(fn, *args, iters, warmup, budget_s):
+    done = ctypes.c_int(0)
+    ms = fn(*args, iters, warmup, ctypes.c_double(budget_s), ctypes.byref(done))
+    return ms, done.value
+
+
+def time_coo(dt, nrows, rowind, colind, vals, x, iters=1000, warmup=2, budget_s=10.0):
+    """Median ms of the COO spmv_host restatement (1 thread), runs done, and
+    the y of the last run."""
+    y = np.zeros(nrows, NP[dt])
+    lib().orc_time_coo.restype = ctypes.c_double
+    ms, done = _timed(lib().orc_time_coo, dt, ctypes.c_uint32(nrows), ctypes.c_uint32(len(vals)), P(rowind),
+                  P(colind), P(vals), P(x), P(y), iters=iters, warmup=warmup, budget_s=budget_s)
+    return ms, done, y
+
+
+def time_bcsr(dt, nrows, ncols, R, C, browptr, bcolind, bval, x, iters=1000, warmup=2, budget_s=10.0):
+    y = np.zeros(nrows, NP[dt])
+    lib().orc_time_bcsr.restype = ctypes.c_double
+    ms, done = _timed(lib().orc_time_bcsr, dt, ctypes.c_uint32(nrows), ctypes.c_uint32(ncols), ctypes.c_uint32(R),
+                  ctypes.c_uint32(C), ctypes.c_uint32(len(browptr) - 1), P(browptr), P(bcolind), P(bval), P(x),
+                  P(y), iters=iters, warmup=warmup, budget_s=budget_s)
+    return ms, done, y
+
+
+def time_bcoo(dt, nrows, ncols, R, C, browind, bcolind, bval, x, iters=1000, warmup=2, budget_s=10.0):
+    y = np.zeros(nrows, NP[dt])
+    lib().orc_time_bcoo.restype = ctypes.c_double
+    ms, done = _timed(lib().orc_time_bcoo, dt, ctypes.c_uint32(nrows), ctypes.c_uint32(ncols), ctypes.c_uint32(R),
+                  ctypes.c_uint32(C), ctypes.c_uint64(len(browind)), P(browind), P(bcolind), P(bval), P(x),
+                  P(y), iters=iters, warmup=warmup, budget_s=budget_s)
+    return ms, done, y
+
+
+def time_tiled(dt, T, nrows, ncols, x, iters=1000, warmup=2, budget_s=10.0):
+    """The 2D tile-major spmv_host restatement on a built tiling (1 thread)."""
+    a = tile_args(T, nrows)
+    u32 = lambda v: np.ascontiguousarray(v, np.uint32)
+    u64 = lambda v: np.ascontiguousarray(v, np.uint64)
+    arrs = [u32(a["row_off"]), u32(a["rows"]), u32(a["col_off"]), u32(a["col_w"]), u64(a["ptr_off"]),
+            u64(a["ent_off"]), u32(a["ptr"]) if a["ptr"] is not None else np.zeros(1, np.uint32),
+            u32(a["ri"]) if a["ri"] is not None else np.zeros(1, np.uint32), u32(a["ci"])]
+    y = np.zeros(nrows, NP[dt])
+    lib().orc_time_tiled.restype = ctypes.c_double
+    ms, done = _timed(lib().orc_time_tiled, dt, T.format, ctypes.c_uint32(nrows), ctypes.c_uint32(ncols),
+                  ctypes.c_uint32(len(arrs[1])), *[P(v) for v in arrs], ctypes.c_uint32(a["R"]),
+                  ctypes.c_uint32(a["C"]), P(a["vals"]), P(x), P(y), iters=iters, warmup=warmup, budget_s=budget_s)
+    return ms, done, y
+
+
+def time_merge(dt, nparts, n, nthreads, iters=1000, warmup=2, budget_s=5.0):
+    """The 2D vertical merge of nparts partial vectors of n values (OpenMP)."""
+    parts = np.zeros((nparts, n), NP[dt])
+    lib().orc_time_merge.restype = ctypes.c_double
+    return _timed(lib().orc_time_merge, dt, ctypes.c_uint32(nparts), ctypes.c_uint64(n), ctypes.c_uint64(n),
+                  P(parts), nthreads, iters=iters, warmup=warmup, budget_s=budget_s)
+
+
 def num_cpus():
     return lib().orc_num_cpus()
 

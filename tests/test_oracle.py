@@ -133,3 +133,28 @@ def test_float32_order_is_sequential_per_row(sp):
         for k in range(C.rowptr[i], C.rowptr[i + 1]):
             acc = np.float32(acc + np.float32(C.values[k] * x[C.colind[k]]))
         assert y[i] == acc, i
+
+
+def test_cpu_baseline_timers_time_the_restatements(sp, orc):
+    """The per-format CPU-baseline timers run the same restatements the
+    parity checks use: the y of a timed run equals the oracle's y."""
+    import numpy as np
+    A = sp.gen_rmat(12, 40000, dtype=2, seed=9)
+    x = sp.gen_vector(2, A.ncols, seed=10)
+    C = A.to_csr()
+    want = orc.spmv_csr(2, C.rowptr, C.colind, C.values, x)
+    ms, done, y = orc.time_coo(2, A.nrows, A.rowind, A.colind, A.values, x, iters=3, warmup=1)
+    assert ms > 0 and done == 3 and np.array_equal(y, want)
+    B = C.to_bcsr(4, 4)
+    ms, done, y = orc.time_bcsr(2, A.nrows, A.ncols, 4, 4, B.browptr, B.bcolind, B.bval, x, iters=3, warmup=1)
+    assert ms > 0 and done == 3 and np.array_equal(y, want)
+    O = B.to_bcoo()
+    ms, done, y = orc.time_bcoo(2, A.nrows, A.ncols, 4, 4, O.browind, O.bcolind, O.bval, x, iters=3, warmup=1)
+    assert ms > 0 and done == 3 and np.array_equal(y, want)
+    for tiling in (0, 1, 2):
+        for fmt in (0, 1, 2, 3):
+            T = sp.coo2tiled(A, tiling, fmt, 4, 4)
+            ms, done, y = orc.time_tiled(2, T, A.nrows, A.ncols, x, iters=2, warmup=0)
+            assert ms > 0 and done == 2 and np.array_equal(y, want), (tiling, fmt)
+    ms, done = orc.time_merge(4, 4, 1000, 2, iters=3, warmup=0)
+    assert ms > 0 and done == 3

@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as orc
+from oracle_lib import tile_args
 
 BITS = {0: 8, 1: 16, 2: 32, 3: 64}
 TILINGS = (0, 1, 2)   # D, RB, BD
@@ -58,34 +59,6 @@ def coo_case(sp, rng, dt, nrows, ncols, nnz, small_ints=False):
         v = rng.uniform(-1, 1, nnz).astype(orc.NP[dt])
         x = rng.uniform(-1, 1, ncols).astype(orc.NP[dt])
     return r, c, v, x, sp.Coo.from_arrays(nrows, ncols, r, c, v)
-
-
-def tile_args(T, nrows):
-    """The tile descriptors orc_spmv_tiled takes, read from a built tiling."""
-    nt, vert = T.ntiles, T.vert_partitions
-    cuts = T.vert_tile_widths.astype(np.int64)
-    t = np.arange(nt)
-    c = t % vert
-    if T.tiling == 0:
-        row_off = (t // vert) * T.tile_height
-    else:
-        row_off = np.zeros(nt, np.int64)
-    rows = np.full(nt, T.tile_height if T.tiling == 0 else nrows, np.int64)
-    col_off, col_w = cuts[c], cuts[c + 1] - cuts[c]
-    if T.format in (0, 1):
-        ptr_off = t * (T.tile_height + 1)
-        ent_off, ci, vals = T.nnz_offset, T.dcolind, T.dval
-        ptr = T.drowptr if T.format == 0 else None
-        ri = T.drowind if T.format == 1 else None
-        R = C = 1
-    else:
-        ptr_off = t * (T.num_block_rows + 1)
-        ent_off, ci, vals = T.block_offset, T.bcolind, T.bval
-        ptr = T.browptr if T.format == 2 else None
-        ri = T.browind if T.format == 3 else None
-        R, C = T.row_block_size, T.col_block_size
-    return dict(row_off=row_off, rows=rows, col_off=col_off, col_w=col_w, ptr_off=ptr_off, ent_off=ent_off,
-                ptr=ptr, ri=ri, ci=ci, R=R, C=C, vals=vals)
 
 
 def tiled_y(dt, T, nrows, ncols, x, **override):
