@@ -981,6 +981,179 @@ __global__ __launch_bounds__(TPB) void rows_rhs_kernel(const Prob *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// BCSR 4 x 4 blocks times K >= 8 right-hand sides on the matrix cores: the
+// block-row product of the reference (spmv/1D/BCSR-nnz/dpu/task.c:337-370)
+// as a dense 4 x 4 x K contraction per block.  A wave takes a block row; its
+// block columns are read 64 at a time, one per lane, and handed out by
+// shuffles.  The matrix arrays are read once per pass for up to 32
+// right-hand sides.
+//
+// FP32, v_mfma_f32_4x4x1_16b_f32: 16 independent 4x4x1 products per
+// instruction; product m = lane / 4 gets A[i] from lane 4m + i and B[j] from
+// lane 4m + j, and lane 4m + j holds D[v][j] in its register v.  Lane
+// l = K*g + 4*b + j takes matrix block g of the step (G = 64 / K blocks per
+// step), right-hand side column 4b + j, and matrix row j: A = block row j,
+// column c; B = x row (4 J + c), column 4b + j.  Four instructions (c = 0..3)
+// form the block's 4 x K contribution; the G partial sums are added across
+// lanes at the end of the row.  A lane reads its block row (A values) as
+// one 16-byte piece and its four B values element by element: the 4K-byte
+// rows of x come in whole, coalesced, with K lanes per row.  (Staging the
+// block values in LDS and reading x in 16-byte pieces with a DPP transpose
+// across each quad measured slower on the 27-point stencil.)
+//
+// FP64, v_mfma_f64_4x4x4_4b_f64: 4 independent 4x4x4 products per
+// instruction; product m takes A[i][k] from lane 16k + 4m + i and B[k][j]
+// from lane 16k + 4m + j, and lane 16i + 4m + j holds D[i][j].  Lane
+// l = 16h + 4m + o supplies block[o][h] (A, from the block values staged in
+// LDS 32 blocks at a time in 16-byte chunks) and x row (4 J + h) (B); the
+// columns that B's j = o stands for are a relabelling of the right-hand
+// sides: with 32 of them, instruction n of a block takes columns
+// 8m + 2o + n, so a lane reads its two B values (and writes its two Y
+// values) as one 16-byte piece.  16: one instruction, columns 4m + o.
+// 8: two blocks per instruction (products 0-1 and 2-3), columns 4(m%2) + o.
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <typename T, int K>
+__global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__ probs,
+                                                        const uint32_t *__restrict__ unit_pid,
+                                                        const uint32_t *__restrict__ urow, Ptrs<T> io,
+                                                        uint32_t nunits) {
+    static_assert(K == 8 || K == 16 || K == 32, "8, 16 or 32 right-hand sides per pass");
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "FP32 or FP64");
+    constexpr int NW = TPB / WAVE;
+    constexpr int NB64 = 32;  // FP64: blocks per LDS batch (4 KB of values per wave)
+    __shared__ __align__(16) double s_val[sizeof(T) == 8 ? NW : 1][sizeof(T) == 8 ? NB64 * 16 : 1];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint32_t pid = unit_pid[u];
+    const Prob P = probs[pid];
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), cip = gp(P.cind);
+    const gptr<const T> __restrict__ bv = gp((const T *)P.val);
+    const int ldx = io.lx(P), ldy = io.ly(P);
+    const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    const uint32_t ncols = P.ncols_elem;
+    for (uint32_t I = r0 + wid; I < r1; I += NW) {
+        const uint32_t b = rp[I], e = rp[I + 1];
+        if constexpr (sizeof(T) == 4) {
+            constexpr int G = WAVE / K;  // matrix blocks per step
+            constexpr int U = 8 / G;     // steps whose loads go out together
+            const int g = lane / K, q = lane % K, j = lane % 4;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            // blocks in batches of 64: one coalesced load of their block columns,
+            // handed to the lanes that need them by a shuffle
+            for (uint32_t base = b; base < e; base += WAVE) {
+                const uint32_t nb = min((uint32_t)WAVE, e - base);
+                const uint32_t Jl = (uint32_t)lane < nb ? ldg(cip + base + lane) : 0u;
+                for (uint32_t t0 = 0; t0 < nb; t0 += G * U) {
+                    T a[U][4], xb[U][4];
+                    bool act[U];
+                    uint32_t J[U];
+#pragma unroll
+                    for (int w = 0; w < U; w++) {
+                        const uint32_t sl = t0 + w * G + g;
+                        act[w] = sl < nb;
+                        const uint32_t sc = act[w] ? sl : 0u;  // a valid block; its product is dropped
+                        J[w] = (uint32_t)__shfl((int)Jl, (int)sc, WAVE);
+                        ldg_n<T, 4>(bv + (size_t)(base + sc) * 16 + j * 4, a[w]);  // matrix row j of the block
+                    }
+#pragma unroll
+                    for (int w = 0; w < U; w++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            const uint32_t col = J[w] * 4 + c;
+                            xb[w][c] = (act[w] && col < ncols) ? x[(size_t)col * ldx + q] : T(0);
+                        }
+#pragma unroll
+                    for (int w = 0; w < U; w++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+                            acc = __builtin_amdgcn_mfma_f32_4x4x1f32(act[w] ? a[w][c] : T(0), xb[w][c], acc, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[v] = xor_sum<K, WAVE>(acc[v]);
+            if (g == 0) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const uint32_t row = I * 4 + v;
+                    if (row < P.nrows_elem) y[(size_t)row * ldy + q] = acc[v];
+                }
+            }
+        } else {
+            constexpr int BPS = K == 8 ? 2 : 1;  // matrix blocks per instruction
+            constexpr int NI = K == 32 ? 2 : 1;  // instructions per block
+            constexpr int U = 4 / BPS;           // steps whose loads go out together
+            double *const sv = s_val[wid];
+            // whole 16-byte pieces of x and y rows (uniform over the launch)
+            const bool xw = ((uintptr_t)io.x(pid, P) % 16 == 0) && (ldx % 2 == 0);
+            const bool yw = ((uintptr_t)io.y(pid, P) % 16 == 0) && (ldy % 2 == 0);
+            const int h = lane / 16, m = (lane / 4) % 4, o = lane % 4;
+            const int gblk = BPS == 2 ? m / 2 : 0;  // block of this product within the step
+            // right-hand side column of instruction n in this lane (a relabelling, see above)
+            auto colq = [&](int n) { return K == 32 ? 8 * m + 2 * o + n : K == 16 ? 4 * m + o : 4 * (m % 2) + o; };
+            double acc[NI];
+#pragma unroll
+            for (int n = 0; n < NI; n++) acc[n] = 0.0;
+            for (uint32_t base = b; base < e; base += NB64) {
+                const uint32_t nb = min((uint32_t)NB64, e - base);
+                // the batch's values to LDS in 16-byte chunks, its block columns to lanes
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t c = lane; c < nb * 8; c += WAVE)
+                    *(u32x4 *)(sv + c * 2) = __builtin_nontemporal_load((gptr<const u32x4>)(bv + (size_t)base * 16) + c);
+                const uint32_t Jl = (uint32_t)lane < nb ? ldg(cip + base + lane) : 0u;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t t0 = 0; t0 < nb; t0 += BPS * U) {
+                    double xv[U][NI];
+                    bool act[U];
+                    uint32_t sidx[U];
+#pragma unroll
+                    for (int w = 0; w < U; w++) {
+                        const uint32_t sl = t0 + w * BPS + gblk;
+                        act[w] = sl < nb;
+                        sidx[w] = act[w] ? sl : 0u;
+                        const uint32_t row = (uint32_t)__shfl((int)Jl, (int)sidx[w], WAVE) * 4 + h;
+                        const bool ok = act[w] && row < ncols;
+                        if (NI == 2 && xw) {
+                            f64x2 v = {0.0, 0.0};
+                            if (ok) v = *((gptr<const f64x2>)(x + (size_t)row * ldx + colq(0)));
+                            xv[w][0] = v[0];
+                            xv[w][NI - 1] = v[1];
+                        } else {
+#pragma unroll
+                            for (int n = 0; n < NI; n++) xv[w][n] = ok ? (double)x[(size_t)row * ldx + colq(n)] : 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int w = 0; w < U; w++) {
+                        const double av = act[w] ? sv[sidx[w] * 16 + o * 4 + h] : 0.0;
+#pragma unroll
+                        for (int n = 0; n < NI; n++) acc[n] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, xv[w][n], acc[n], 0, 0, 0);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();  // the next batch overwrites the staged values
+            }
+            const uint32_t row = I * 4 + h;
+            if constexpr (BPS == 2) acc[0] += lane_xor<8>(acc[0]);  // the other block of the step: m ^ 2
+            if ((BPS == 1 || gblk == 0) && row < P.nrows_elem) {
+                if (NI == 2 && yw) {
+                    const f64x2 v = {acc[0], acc[NI - 1]};
+                    *((gptr<f64x2>)(y + (size_t)row * ldy + colq(0))) = v;
+                } else {
+#pragma unroll
+                    for (int n = 0; n < NI; n++) y[(size_t)row * ldy + colq(n)] = (T)acc[n];
+                }
+            }
+        }
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // BCOO, block balanced: phase 1 forms the R row sums of every block of the
 // tile with lane groups of C over the block's elements (coalesced value
 // loads); phase 2 is the COO segmented walk over blocks with R-wide sums.

@@ -27,6 +27,12 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
             return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
         }
         if (w.kind == WK_BCSR_VEC) {
+            if constexpr (__is_same(T, float) || __is_same(T, double)) {
+                if (w.R == 4 && w.C == 4) {  // 4 x 4 blocks: the matrix-core kernel
+                    hipLaunchKernelGGL((bcsr_mfma_kernel<T, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits);
+                    return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+                }
+            }
             if constexpr (sizeof(T) == 8) {
                 // 8-byte values: lanes across the right-hand sides (R sums per lane) beat R*K
                 // register-blocked sums (stencil FP64 4x4, 32 columns: 1.57 vs 2.20 ms)
@@ -137,6 +143,12 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         }
         break;
     case WK_BCSR_VEC: {
+        if constexpr (K == 8 && (__is_same(T, float) || __is_same(T, double))) {
+            if (w.R == 4 && w.C == 4) {  // 4 x 4 blocks: the matrix-core kernel
+                hipLaunchKernelGGL((bcsr_mfma_kernel<T, 8>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits);
+                break;
+            }
+        }
         if (w.lanes == 0 && K == 1) {
             // coalesced kernel (the plan checked the shape against the value size)
 #define SPD_CO(Rv, Cv) \

@@ -41,7 +41,8 @@ struct Work {
 };
 
 // Widest right-hand-side chunk a kernel kind takes in one pass: the
-// row-granular CSR kernel keeps K accumulators per lane; the register-blocked
+// row-granular CSR kernel keeps K accumulators per lane; FP32/FP64 4 x 4
+// BCSR blocks run on the matrix cores (8, 16 or 32 per pass); the register-blocked
 // BCSR kernel R*K (R*K <= 64); for 8-byte values and R*C <= 16, lanes run
 // across up to 32 right-hand sides with R sums each; the kernels with carries
 // keep K partial sums per carry slot.
@@ -49,7 +50,9 @@ inline int max_rhs_chunk(const Work &w) {
     const bool wide8 = (w.dtype == SP_I64 || w.dtype == SP_F64) && w.R * w.C <= 16;
     switch (w.kind) {
     case WK_CSR_VEC: return 32;
-    case WK_BCSR_VEC: return wide8 ? 32 : w.R <= 2 ? 32 : w.R <= 4 ? 16 : 8;
+    case WK_BCSR_VEC:
+        if ((w.dtype == SP_F32 || w.dtype == SP_F64) && w.R == 4 && w.C == 4) return 32;  // matrix-core kernel
+        return wide8 ? 32 : w.R <= 2 ? 32 : w.R <= 4 ? 16 : 8;
     case WK_BCOO_SEG: return 1;
     case WK_BCSR_SEG: return 1;
     case WK_CSR_HOT: return 1;

@@ -253,3 +253,20 @@ def test_coo_strided_output(sp, ctx, dt):
         assert_parity_multi(dt, np.ascontiguousarray(Y[:, :k]), A, X, kern)
         pad = np.ascontiguousarray(Y[:, k:]).view(np.uint8)
         assert np.array_equal(pad, np.full(pad.shape, 0x5A, np.uint8)), kern
+
+
+@pytest.mark.parametrize("dt", [4, 5])
+@pytest.mark.parametrize("k", [8, 16, 24, 32, 40, 64])
+def test_bcsr4_spmm_matrix_cores(sp, ctx, dt, k):
+    """4x4 BCSR blocks times 8/16/32-column chunks run on the matrix cores:
+    dense 4x4 blocks (4 unknowns per grid point), a truncated last block row
+    and block column (rows and columns not multiples of 4), empty block rows,
+    1D plans of both balances, 2D tilings, and every chunk size."""
+    cases = [sp.gen_stencil27(7, 4, True, dt, seed=7),
+             sp.gen_uniform(1001, 998, 0, 30, dtype=dt, seed=8)]
+    for A in cases:
+        X = np.stack([sp.gen_vector(dt, A.ncols, seed=80 + q) for q in range(k)], axis=1)
+        for kern in ("BCSR-block", "BCSR-nnz", "DBCSR", "RBDBCSR-nnz", "BDBCSR-block"):
+            kw = dict(vert=3) if kern[0] in "DRB" and kern[1] != "C" else {}
+            Y, _ = run(sp, ctx, kern, A, X, sp.opts(row_block_size=4, col_block_size=4, **kw), k=k)
+            assert_parity_multi(dt, Y, A, X, f"{kern} k{k} n{A.nrows}")
