@@ -3,8 +3,10 @@
 // The problems of all plans are concatenated into launch groups keyed by
 // (kernel kind, tile shape, right-hand-side chunk width); each group is one
 // persistent launch whose workgroups sweep every problem's work units, with
-// x / y taken from a per-call device pointer table.  The groups and their
-// carry fix-ups are captured once into a hipGraph and replayed.  There is
+// x / y taken from a per-call device pointer table.  Groups are independent
+// (each writes its own problems' columns), so they run on up to four
+// parallel branches.  The groups and their carry fix-ups are captured once
+// into a hipGraph and replayed.  There is
 // no counterpart in the reference (which runs one matrix per launch).
 #include <string.h>
 
@@ -67,6 +69,13 @@ struct sp_batch {
     hipStream_t captured_on = nullptr;
     uint64_t algo_bytes = 0;
     std::vector<void *> allocs;
+    // independent groups run on parallel branches (streams forked from and
+    // joined back into the context stream), so one group's tail overlaps the
+    // others' work; branch[i] runs the groups with group_branch == i
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
+    std::vector<int> group_branch;
 };
 
 static int balloc(sp_batch *b, size_t bytes, void **out) {
@@ -84,6 +93,9 @@ template <typename V> static int bupload(sp_batch *b, const std::vector<V> &v, V
 
 static void batch_free(sp_batch *b) {
     if (!b) return;
+    for (hipStream_t st : b->branch) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : b->join) (void)hipEventDestroy(ev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->tab_done) (void)hipEventDestroy(b->tab_done);
@@ -211,6 +223,31 @@ extern "C" int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, co
                 rc = SP_ERR_HIP;
         }
     }
+    if (!rc && b->groups.size() > 1) {
+        // up to kBranches branches (default-priority streams), groups assigned
+        // largest first to the branch with the least work so far
+        constexpr size_t kBranches = 4;
+        const size_t nbr = std::min(kBranches, b->groups.size());
+        std::vector<size_t> order(b->groups.size());
+        for (size_t gi = 0; gi < order.size(); gi++) order[gi] = gi;
+        auto work = [&](size_t gi) { return (uint64_t)b->groups[gi].w.nunits * (uint64_t)b->groups[gi].kc; };
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return work(a) > work(c); });
+        std::vector<uint64_t> load(nbr, 0);
+        b->group_branch.assign(b->groups.size(), 0);
+        for (size_t gi : order) {
+            const size_t k2 = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            b->group_branch[gi] = (int)k2;
+            load[k2] += work(gi);
+        }
+        b->branch.resize(nbr, nullptr);
+        b->join.resize(nbr, nullptr);
+        for (size_t i = 0; i < nbr && !rc; i++) {
+            if (hipStreamCreateWithFlags(&b->branch[i], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess)
+                rc = SP_ERR_HIP;
+        }
+        if (!rc && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) rc = SP_ERR_HIP;
+    }
     if (rc) { batch_free(b); return rc; }
     *out = b;
     return SP_OK;
@@ -244,24 +281,43 @@ static int batch_tables(sp_batch *b, const void *const *X, void *const *Y) {
     return SP_OK;
 }
 
-static int batch_launch(sp_batch *b) {
+// Every group, on the context stream or forked over the branches and joined
+// back into it (also while the context stream is being captured).
+static int batch_enqueue(sp_batch *b) {
     sp_ctx *c = b->ctx;
-    if (!(b->flags & SP_BATCH_GRAPH)) {
+    if (b->branch.empty()) {
         for (const Group &g : b->groups) {
             int rc = run_group(g, c, b->d_tab);
             if (rc) return rc;
         }
         return SP_OK;
     }
+    SP_HIP_TRY(hipEventRecord(b->fork, c->stream));
+    for (hipStream_t st : b->branch) SP_HIP_TRY(hipStreamWaitEvent(st, b->fork, 0));
+    int rc = SP_OK;
+    for (size_t gi = 0; gi < b->groups.size() && !rc; gi++) {
+        sp_ctx cb = *c;  // the same context, launching on the group's branch
+        cb.stream = b->branch[b->group_branch[gi]];
+        cb.mark_mid = false;
+        rc = run_group(b->groups[gi], &cb, b->d_tab);
+    }
+    // join every branch, also after a failed launch, so a capture can end
+    for (size_t i = 0; i < b->branch.size(); i++) {
+        if (hipEventRecord(b->join[i], b->branch[i]) != hipSuccess ||
+            hipStreamWaitEvent(c->stream, b->join[i], 0) != hipSuccess)
+            rc = rc ? rc : SP_ERR_HIP;
+    }
+    return rc;
+}
+
+static int batch_launch(sp_batch *b) {
+    sp_ctx *c = b->ctx;
+    if (!(b->flags & SP_BATCH_GRAPH)) return batch_enqueue(b);
     if (!b->gexec || b->captured_on != c->stream) {
         if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
         if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
         SP_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-        int rc = SP_OK;
-        for (const Group &g : b->groups) {
-            rc = run_group(g, c, b->d_tab);
-            if (rc) break;
-        }
+        const int rc = batch_enqueue(b);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }

@@ -86,6 +86,16 @@ template <int L, typename A> __device__ __forceinline__ A group_sum(A v) {
     return v;
 }
 
+// v += lane_xor<M>(v) for every power-of-two M in [lo, hi)
+template <int M, int HI, typename A> __device__ __forceinline__ A xor_sum(A v) {
+    if constexpr (M < HI) {
+        v += lane_xor<M>(v);
+        return xor_sum<M * 2, HI>(v);
+    } else {
+        return v;
+    }
+}
+
 constexpr int WAVE = 64;
 
 // Uncached-at-L2-for-reuse hints are not used: every matrix stream is read

@@ -547,6 +547,70 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// CSR, row granular, K >= 8 right-hand sides: lanes across the right-hand
+// sides.  A wave takes a row; lane l = K*g + q works on column q of entry
+// slot g (G = 64 / K entries of the row per step).  The row's column
+// indices and values are read 64 at a time, one entry per lane, and handed
+// out by shuffles, so each step is one coalesced read of G rows of x (K
+// consecutive values each) and one multiply-add per lane; the G partial
+// sums are added across lanes at the end of the row.  The row loop of the
+// reference tasklet (spmv/1D/CSR-row/dpu/task.c:105-202) with the
+// right-hand sides spread over lanes instead of entries, so no per-column
+// reduction over many lanes is needed.  Used where a row of x is 64 bytes or
+// more (K * sizeof(T) >= 64); narrower rows keep csr_vector_kernel, whose
+// lanes read whole x rows in 16-byte pieces (FP32, 8 columns on the 27-point
+// stencil: 532 against 667 us).
+// ---------------------------------------------------------------------------
+template <typename T, int K>
+__global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ probs,
+                                                      const uint32_t *__restrict__ unit_pid,
+                                                      const uint32_t *__restrict__ urow, Ptrs<T> io,
+                                                      uint32_t nunits) {
+    static_assert(K == 8 || K == 16 || K == 32, "8, 16 or 32 right-hand sides per pass");
+    using A = acc_t<T>;
+    constexpr int NW = TPB / WAVE, G = WAVE / K, U = 4;
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    const int g = lane / K, q = lane % K;
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint32_t pid = unit_pid[u];
+    const Prob P = probs[pid];
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), colind = gp(P.cind);
+    const gptr<const T> __restrict__ vals = gp((const T *)P.val);
+    const int ldx = io.lx(P), ldy = io.ly(P);
+    const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    for (uint32_t r = r0 + wid; r < r1; r += NW) {
+        const uint32_t b = rp[r], e = rp[r + 1];
+        A acc = A(0);
+        for (uint32_t base = b; base < e; base += WAVE) {
+            const uint32_t nb = min((uint32_t)WAVE, e - base);
+            const uint32_t cl = (uint32_t)lane < nb ? ldg(colind + base + lane) : 0u;
+            const T vl = (uint32_t)lane < nb ? ldg(vals + base + lane) : T(0);
+            for (uint32_t t0 = 0; t0 < nb; t0 += G * U) {
+                T v[U], xv[U];
+#pragma unroll
+                for (int w = 0; w < U; w++) {
+                    const uint32_t s = t0 + w * G + g;
+                    const bool act = s < nb;
+                    const int src = act ? (int)s : 0;
+                    // every lane takes part in the shuffles (a lane reads no value from an inactive one)
+                    const uint32_t c = (uint32_t)__shfl((int)cl, src, WAVE);
+                    const T vs = (T)__shfl(vl, src, WAVE);
+                    v[w] = act ? vs : T(0);
+                    xv[w] = act ? x[(size_t)c * ldx + q] : T(0);
+                }
+#pragma unroll
+                for (int w = 0; w < U; w++) acc = fma_acc<T>(v[w], xv[w], acc);
+            }
+        }
+        acc = xor_sum<K, WAVE>(acc);
+        if (g == 0) y[(size_t)r * ldy + q] = to_val<T>(acc);
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // COO, entry balanced with a segmented reduction.  Entries are sorted by
 // row.  A thread owns IPT consecutive entries; a row is written by the
 // thread holding its last entry, with the partial sums of earlier threads
@@ -804,16 +868,6 @@ template <typename T, int R, int C> struct Coal {
     static constexpr int PB = WAVE / G;             // blocks per wave step
     static constexpr bool ok = (C % CH == 0) && G <= WAVE && (WAVE % G == 0);
 };
-
-// v += lane_xor<M>(v) for every power-of-two M in [lo, hi)
-template <int M, int HI, typename A> __device__ __forceinline__ A xor_sum(A v) {
-    if constexpr (M < HI) {
-        v += lane_xor<M>(v);
-        return xor_sum<M * 2, HI>(v);
-    } else {
-        return v;
-    }
-}
 
 template <typename T, int R, int C>
 __global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__ probs,

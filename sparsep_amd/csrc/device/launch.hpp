@@ -18,12 +18,16 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
     if constexpr (K > 8) {
         // wide chunks: row-granular kernels only (no carries, no LDS staging)
         if (w.kind == WK_CSR_VEC) {
+            if constexpr (K * sizeof(T) >= 64) {  // x rows of 64 bytes or more: lanes across the right-hand sides
+                hipLaunchKernelGGL((csr_rhs_kernel<T, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits);
+            } else {
 #define SPD_VECW(Lv) case Lv: hipLaunchKernelGGL((csr_vector_kernel<T, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
-            switch (w.lanes) {
-                SPD_VECW(1) SPD_VECW(2) SPD_VECW(4) SPD_VECW(8) SPD_VECW(16) SPD_VECW(32) SPD_VECW(64)
-            default: return SP_ERR_ARG;
-            }
+                switch (w.lanes) {
+                    SPD_VECW(1) SPD_VECW(2) SPD_VECW(4) SPD_VECW(8) SPD_VECW(16) SPD_VECW(32) SPD_VECW(64)
+                default: return SP_ERR_ARG;
+                }
 #undef SPD_VECW
+            }
             return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
         }
         if (w.kind == WK_BCSR_VEC) {
@@ -96,6 +100,10 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         break;
     }
     case WK_CSR_VEC:
+        if constexpr (K * sizeof(T) >= 64) {  // x rows of 64 bytes or more: lanes across the right-hand sides
+            hipLaunchKernelGGL((csr_rhs_kernel<T, 8>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits);
+            break;
+        }
 #define SPD_VEC(Lv) case Lv: hipLaunchKernelGGL((csr_vector_kernel<T, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
         switch (w.lanes) {
             SPD_VEC(1) SPD_VEC(2) SPD_VEC(4) SPD_VEC(8) SPD_VEC(16) SPD_VEC(32) SPD_VEC(64)
