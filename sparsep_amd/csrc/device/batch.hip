@@ -150,10 +150,12 @@ extern "C" int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, co
                 g.w.lanes = w.lanes;
                 g.w.R = w.R;
                 g.w.C = w.C;
+                g.w.chunk = w.chunk;
             } else {
                 gi = it->second;
             }
             Group &g = b->groups[gi];
+            g.w.chunk = g.w.chunk && w.chunk;
             Acc &a = acc[gi];
             const uint32_t pid0 = (uint32_t)a.probs.size();
             for (const spd::Prob &P0 : p->h_probs) {

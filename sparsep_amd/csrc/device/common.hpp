@@ -128,6 +128,9 @@ template <typename T, int N> __device__ __forceinline__ void ldg_n(gptr<const T>
 #pragma unroll
         for (int i = 0; i < B / 8; i++) w[i] = __builtin_nontemporal_load((gptr<const u32x2>)p + i);
         __builtin_memcpy(out, w, B);
+    } else if constexpr (B == 4) {
+        const uint32_t w = __builtin_nontemporal_load((gptr<const uint32_t>)p);
+        __builtin_memcpy(out, &w, B);
     } else {
 #pragma unroll
         for (int i = 0; i < N; i++) out[i] = ldg(p + i);

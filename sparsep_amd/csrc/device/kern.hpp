@@ -136,6 +136,20 @@ __device__ __host__ inline uint32_t merge_search(uint32_t d, uint32_t na, uint32
     return lo;
 }
 
+// Four consecutive entries a .. a+3 of a (column index, value) stream (a a
+// multiple of 4, both arrays aligned to 4 entries and readable 3 entries
+// past their end): one 16-byte index load and one 4 * sizeof(T) value load.
+template <typename T>
+__device__ __forceinline__ void load4(gptr<const uint32_t> ci, gptr<const T> va, uint32_t a, uint32_t (&c)[4],
+                                      T (&v)[4]) {
+    const u32x4 w = __builtin_nontemporal_load((gptr<const u32x4>)(ci + a));
+    c[0] = w[0];
+    c[1] = w[1];
+    c[2] = w[2];
+    c[3] = w[3];
+    ldg_n<T, 4>(va + a, v);
+}
+
 // LDS bytes for a staged x slice (2D tiles): three workgroups per CU fit
 // next to the tile buffers.
 constexpr int SLICE_BYTES = 32768;
@@ -146,8 +160,8 @@ constexpr int SLICE_BYTES = 32768;
 // current tile (its ncols_elem columns) into LDS when the tile changes, and
 // gathers from LDS; the sums are the same as without the slice.
 // ---------------------------------------------------------------------------
-template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB>
-__global__ __launch_bounds__(NT) void csr_merge_kernel(const Prob *__restrict__ probs,
+template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB, bool CH = false>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH ? 6 : 1))) void csr_merge_kernel(const Prob *__restrict__ probs,
                                                         const uint32_t *__restrict__ unit_pid,
                                                         const uint32_t *__restrict__ crow,
                                                         const uint32_t *__restrict__ cnnz, Ptrs<T> io,
@@ -205,37 +219,92 @@ __global__ __launch_bounds__(NT) void csr_merge_kernel(const Prob *__restrict__ 
     // All loads of the tile are issued before the first is consumed, then
     // all x gathers: two memory round trips per tile.
     const uint32_t nrl = min(na + 1, P.nrows - i0);
-    uint32_t re[IPT + 1], cv[IPT];
-    T vv[IPT];
+    uint32_t re[IPT + 1];
 #pragma unroll
     for (int q = 0; q <= IPT; q++) {
         const uint32_t k = tid + q * NT;
         if (k < nrl) re[q] = rp[i0 + k + 1];
     }
-#pragma unroll
-    for (int q = 0; q < IPT; q++) {
-        const uint32_t k = tid + q * NT;
-        if (k < nb) {
-            cv[q] = ldg(cip + j0 + k);
-            vv[q] = ldg(vals + j0 + k);
+    // CH (1- and 2-byte values; the plan's arrays aligned to 4 entries and
+    // padded past their end, as every 1D plan's): entries come in 4-entry
+    // chunks aligned in the arrays, one 16-byte index load and one
+    // 4 * sizeof(T) value load each, so the values stream by bytes.  Chunk c
+    // covers array entries a0 + 4c .. a0 + 4c + 3, of which those in
+    // [j0, j0 + nb) are the tile's; thread tid takes chunks tid + r NT, and
+    // the last thread one more, afterwards, when the tile's entries straddle
+    // a further chunk.  Six waves per SIMD are kept (80 registers).  (R-MAT
+    // 2^20, kernel time: INT8 78.5 -> 66.7 us, INT16 85.5 -> 72.8 us; for 4-
+    // and 8-byte values the lane-strided layout below is faster, 81 against
+    // 97 us per FP32 SpMV, because one gather instruction then covers 64
+    // consecutive entries of a column-ordered tile instead of every fourth
+    // of 256.)  Otherwise: one entry per lane and load, entries tid + q NT.
+    // Either way each product goes to its entry's slot.
+    constexpr bool chunked = CH;
+    auto store = [&](uint32_t e, uint32_t cw, T vw, T xw) {
+        const uint32_t pos = pb ? (cw & pmask) : e;
+        if constexpr (PROD) {
+            s_prod[pos] = mul<T>(vw, xw);
+        } else {
+            s_col[pos] = cw >> pb;
+            s_val[pos] = vw;
         }
-    }
-    if constexpr (PROD) {
-        T xv[IPT];
+    };
+    auto gather = [&](uint32_t cw) -> T {
+        if constexpr (PROD) return SLICE ? s_x[cw >> pb] : x[(size_t)(cw >> pb) * ldx];
+        else return T(0);
+    };
+    if constexpr (chunked) {
+        constexpr int R = IPT / 4;
+        static_assert(IPT % 4 == 0, "whole chunks per thread");
+        const uint32_t a0 = j0 & ~3u, lead = j0 - a0, nch = (lead + nb + 3) >> 2;
+        uint32_t cv[R][4];
+        T vv[R][4], xv[R][4];
 #pragma unroll
-        for (int q = 0; q < IPT; q++)
-            if (tid + q * NT < nb) xv[q] = SLICE ? s_x[cv[q] >> pb] : x[(size_t)(cv[q] >> pb) * ldx];
+        for (int r = 0; r < R; r++) {
+            const uint32_t c = tid + r * NT;
+            if (c < nch) load4<T>(cip, vals, a0 + 4 * c, cv[r], vv[r]);
+        }
 #pragma unroll
-        for (int q = 0; q < IPT; q++)
-            if (tid + q * NT < nb) s_prod[pb ? (cv[q] & pmask) : tid + q * NT] = mul<T>(vv[q], xv[q]);
-    } else {
+        for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int q = 0; q < IPT; q++)
-            if (tid + q * NT < nb) {
-                const uint32_t pos = pb ? (cv[q] & pmask) : tid + q * NT;
-                s_col[pos] = cv[q] >> pb;
-                s_val[pos] = vv[q];
+            for (int i = 0; i < 4; i++) {
+                const uint32_t t4 = 4 * (tid + r * NT) + i;  // lead + tile entry
+                if (t4 >= lead && t4 - lead < nb) xv[r][i] = gather(cv[r][i]);
             }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t t4 = 4 * (tid + r * NT) + i;
+                if (t4 >= lead && t4 - lead < nb) store(t4 - lead, cv[r][i], vv[r][i], xv[r][i]);
+            }
+        if (tid == NT - 1 && (uint32_t)(R * NT) < nch) {  // one more chunk: lead + nb > 4 R NT
+            uint32_t cw[4];
+            T vw[4];
+            load4<T>(cip, vals, a0 + 4 * (uint32_t)(R * NT), cw, vw);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t t4 = 4 * (uint32_t)(R * NT) + i;
+                if (t4 - lead < nb) store(t4 - lead, cw[i], vw[i], gather(cw[i]));
+            }
+        }
+    } else {
+        uint32_t cv[IPT];
+        T vv[IPT], xv[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const uint32_t k = tid + q * NT;
+            if (k < nb) {
+                cv[q] = ldg(cip + j0 + k);
+                vv[q] = ldg(vals + j0 + k);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * NT < nb) xv[q] = gather(cv[q]);
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * NT < nb) store(tid + q * NT, cv[q], vv[q], xv[q]);
     }
 #pragma unroll
     for (int q = 0; q <= IPT; q++)
@@ -616,8 +685,8 @@ __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ p
 // thread holding its last entry, with the partial sums of earlier threads
 // of the tile added in; partials of earlier tiles arrive in the fix-up.
 // ---------------------------------------------------------------------------
-template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB, bool ATOM = false>
-__global__ __launch_bounds__(NT) void coo_seg_kernel(const Prob *__restrict__ probs,
+template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB, bool ATOM = false, bool CH = false>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH ? 6 : 1))) void coo_seg_kernel(const Prob *__restrict__ probs,
                                                       const uint32_t *__restrict__ unit_pid,
                                                       const uint32_t *__restrict__ cent, Ptrs<T> io,
                                                       uint32_t *__restrict__ carry_row,
@@ -664,37 +733,89 @@ __global__ __launch_bounds__(NT) void coo_seg_kernel(const Prob *__restrict__ pr
     // tiles stored in column order carry each entry's row-order position in the low bits
     const uint32_t pb = P.pbits, pmask = (1u << pb) - 1;
 
-    uint32_t rv[IPT], cv[IPT];
-    T vv[IPT];
-#pragma unroll
-    for (int q = 0; q < IPT; q++) {
-        const uint32_t k = tid + q * NT;
-        if (k < n) {
-            rv[q] = ldg(rip + e0 + k);
-            cv[q] = ldg(cip + e0 + k);
-            vv[q] = ldg(vals + e0 + k);
+    // tile entry e: its row to s_row[e], its product (or column and value)
+    // to its row-order slot
+    auto store = [&](uint32_t e, uint32_t rw, uint32_t cw, T vw, T xw) {
+        const uint32_t pos = pb ? (cw & pmask) : e;
+        if constexpr (PROD) {
+            s_prod[pos] = mul<T>(vw, xw);
+        } else {
+            s_col[pos] = cw >> pb;
+            s_val[pos] = vw;
         }
-    }
-    if constexpr (PROD) {
-        T xv[IPT];
+        s_row[e] = rw;
+    };
+    auto gather = [&](uint32_t cw) -> T {
+        if constexpr (PROD) return SLICE ? s_x[cw >> pb] : x[(size_t)(cw >> pb) * ldx];
+        else return T(0);
+    };
+    if constexpr (CH) {
+        // the plan's arrays aligned to 4 entries and padded past their end: 4-entry
+        // chunks aligned in the arrays (16-byte row and column loads, one
+        // 4 * sizeof(T) value load), as in csr_merge_kernel; used for 1- and
+        // 2-byte values (R-MAT 2^20: INT8 83.6 -> 68.8 us, INT16 90.3 -> 79.0 us)
+        constexpr int R = IPT / 4;
+        static_assert(IPT % 4 == 0, "whole chunks per thread");
+        const uint32_t a0 = e0 & ~3u, lead = e0 - a0, nch = (lead + n + 3) >> 2;
+        uint32_t rv[R][4], cv[R][4];
+        T vv[R][4], xv[R][4];
 #pragma unroll
-        for (int q = 0; q < IPT; q++)
-            if (tid + q * NT < n) xv[q] = SLICE ? s_x[cv[q] >> pb] : x[(size_t)(cv[q] >> pb) * ldx];
-#pragma unroll
-        for (int q = 0; q < IPT; q++)
-            if (tid + q * NT < n) s_prod[pb ? (cv[q] & pmask) : tid + q * NT] = mul<T>(vv[q], xv[q]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < IPT; q++)
-            if (tid + q * NT < n) {
-                const uint32_t pos = pb ? (cv[q] & pmask) : tid + q * NT;
-                s_col[pos] = cv[q] >> pb;
-                s_val[pos] = vv[q];
+        for (int r = 0; r < R; r++) {
+            const uint32_t c = tid + r * NT;
+            if (c < nch) {
+                const u32x4 w = __builtin_nontemporal_load((gptr<const u32x4>)(rip + a0 + 4 * c));
+                rv[r][0] = w[0];
+                rv[r][1] = w[1];
+                rv[r][2] = w[2];
+                rv[r][3] = w[3];
+                load4<T>(cip, vals, a0 + 4 * c, cv[r], vv[r]);
             }
-    }
+        }
 #pragma unroll
-    for (int q = 0; q < IPT; q++)
-        if (tid + q * NT < n) s_row[tid + q * NT] = rv[q];
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t t4 = 4 * (tid + r * NT) + i;  // lead + tile entry
+                if (t4 >= lead && t4 - lead < n) xv[r][i] = gather(cv[r][i]);
+            }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t t4 = 4 * (tid + r * NT) + i;
+                if (t4 >= lead && t4 - lead < n) store(t4 - lead, rv[r][i], cv[r][i], vv[r][i], xv[r][i]);
+            }
+        if (tid == NT - 1 && (uint32_t)(R * NT) < nch) {  // one more chunk: lead + n > 4 R NT
+            const uint32_t a = a0 + 4 * (uint32_t)(R * NT);
+            const u32x4 w = __builtin_nontemporal_load((gptr<const u32x4>)(rip + a));
+            uint32_t cw[4];
+            T vw[4];
+            load4<T>(cip, vals, a, cw, vw);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t t4 = 4 * (uint32_t)(R * NT) + i;
+                if (t4 - lead < n) store(t4 - lead, w[i], cw[i], vw[i], gather(cw[i]));
+            }
+        }
+    } else {
+        uint32_t rv[IPT], cv[IPT];
+        T vv[IPT], xv[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const uint32_t k = tid + q * NT;
+            if (k < n) {
+                rv[q] = ldg(rip + e0 + k);
+                cv[q] = ldg(cip + e0 + k);
+                vv[q] = ldg(vals + e0 + k);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * NT < n) xv[q] = gather(cv[q]);
+#pragma unroll
+        for (int q = 0; q < IPT; q++)
+            if (tid + q * NT < n) store(tid + q * NT, rv[q], cv[q], vv[q], xv[q]);
+    }
     __syncthreads();
 
     const uint32_t a = min((uint32_t)tid * IPT, n), b = min(a + IPT, n);

@@ -83,6 +83,19 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
                 }
                 break;
             }
+            // 1- and 2-byte values, arrays aligned to 4 entries and padded: 4-entry chunk loads
+            // (4- and 8-byte values keep lane-strided loads, whose gathers share more lines)
+            if constexpr (sizeof(T) <= 2) {
+            if (w.chunk) {
+                switch (w.ipt) {
+                case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, 1, false, TPB, true>), grid, block, 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+                case 8: hipLaunchKernelGGL((csr_merge_kernel<T, 8, 1, false, TPB, true>), grid, block, 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+                case 16: hipLaunchKernelGGL((csr_merge_kernel<T, 8, 1, false, 2 * TPB, true>), grid, dim3(2 * TPB), 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
+                default: return SP_ERR_ARG;
+                }
+                break;
+            }
+            }
         }
         switch (w.ipt) {
         case 4: hipLaunchKernelGGL((csr_merge_kernel<T, 4, K>), grid, block, 0, s, w.d_prob, upid, w.d_c0, w.d_c1, io, w.d_carry_row, cv, w.nunits); break;
@@ -134,6 +147,17 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
                 default: return SP_ERR_ARG;
                 }
                 break;
+            }
+            if constexpr (sizeof(T) <= 2) {  // as for CSR above
+            if (w.chunk) {
+                switch (w.ipt) {
+                case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, 1, false, TPB, false, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+                case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, 1, false, TPB, false, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+                case 16: hipLaunchKernelGGL((coo_seg_kernel<T, 8, 1, false, 2 * TPB, false, true>), grid, dim3(2 * TPB), 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+                default: return SP_ERR_ARG;
+                }
+                break;
+            }
             }
         }
         switch (w.ipt) {

@@ -464,6 +464,7 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
     p->nparts = p->ntiles = crow.empty() ? 0 : (uint32_t)crow.size() - 1;
     p->nsub = TPB;
     b.add(P, crow, &cnnz);
+    p->work.chunk = true;  // the plan's own arrays: aligned and padded (plan_alloc)
     return b.finish(WK_CSR_MERGE, true, 1);
 }
 
@@ -581,6 +582,7 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
         for (uint32_t i = 0; i < n; i++)
             if (rowptr[i + 1] == rowptr[i]) { b.zpid.push_back(0); b.zrow.push_back(i); }
         b.add(P, units, nullptr);
+        p->work.chunk = true;  // the plan's own arrays: aligned and padded (plan_alloc)
         rc = b.finish(WK_COO_SEG, true, 1);
         if (!rc && atom) rc = atomic_prezero(p, A->rowind, units, 1, n, b.zrow);
     } while (0);

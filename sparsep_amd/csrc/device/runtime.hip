@@ -129,9 +129,11 @@ extern "C" int sp_memcpy_d2h(sp_ctx *c, void *dst, const void *src, size_t bytes
 // Plan
 // ---------------------------------------------------------------------------
 
+// Every plan array gets at least 32 bytes of slack past its end, so kernels may
+// read whole 4-entry chunks of an index or value array (up to 3 entries, 24
+// bytes, past the last one) without leaving the allocation.
 int plan_alloc(sp_plan *p, size_t bytes, void **out) {
-    if (bytes == 0) bytes = 16;
-    bytes = (bytes + 255) & ~(size_t)255;
+    bytes = (bytes + 32 + 255) & ~(size_t)255;
     if (p->opts.mem_cap_bytes && p->device_bytes + bytes > p->opts.mem_cap_bytes) return SP_ERR_NOMEM;
     void *d = nullptr;
     if (hipMalloc(&d, bytes) != hipSuccess) return SP_ERR_NOMEM;

@@ -36,6 +36,8 @@ struct Work {
     spd::Run *d_runs = nullptr;  // CSR merge: row runs that cross tiles (fix-up list), nruns of them
     uint32_t nruns = 0;
     bool atomic = false;         // COO/BCOO: rows crossing tiles merged by atomics (no carries)
+    bool chunk = false;          // CSR merge / COO: every problem's arrays aligned to 4 entries and padded
+                                 // past their end (library allocations), so tiles load 4-entry chunks
     uint32_t nprez = 0;          // atomic: rows zeroed before the main launch
     uint32_t *d_prez_pid = nullptr, *d_prez_row = nullptr;
 };
