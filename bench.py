@@ -454,12 +454,10 @@ def wl_batched(sp, ctx, args, orc):
                 A = sp.gen_rmat(scale, (1 << scale) * 33, dtype=dt, seed=seed)
             t0 = time.perf_counter()
             pts = sp.tune_space(A, sp.KID["CSR-nnz"])
-            sp.tune(ctx, A, pts, iters=3)
+            sp.tune(ctx, A, pts, iters=3, k=k)  # timed with the problem's own number of columns
             best = sp.tune_best(pts)
             tune_s += time.perf_counter() - t0
             o = sp.point_opts(pts[best]) if best >= 0 else None
-            if k > 1 and o is not None and o.algo == sp.ALGO_SEGMENTED and o.items_per_lane > 8:
-                o.items_per_lane = 8  # the multi-RHS merge kernel takes at most 8 items per lane
             mats.append(A)
             plans.append(sp.Plan(ctx, "CSR-nnz", A, o))
             ks.append(k)

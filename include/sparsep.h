@@ -589,6 +589,12 @@ typedef struct sp_tune_point {
 int sp_tune_space(const sp_coo *A, int kernel_id, sp_tune_point *pts, int cap, int *count);
 int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters,
                 uint64_t flush_bytes);
+/* sp_tune_run for Y = A X with k right-hand sides (X and Y row-major, ld = k):
+ * the points are timed with sp_spmm, so a multi-RHS workload is tuned for its
+ * own width (the kernels differ in how many right-hand sides one pass over
+ * the matrix takes). */
+int sp_tune_run_spmm(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters,
+                     uint64_t flush_bytes, int k);
 int sp_tune_best(const sp_tune_point *pts, int n, int kernel_id);
 int sp_tune_db_append(const char *path, const char *tag, const sp_tune_point *pts, int n);
 

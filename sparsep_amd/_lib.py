@@ -167,6 +167,7 @@ SIGNATURES = {
     "sp_time_copy": (I, [P, U64, I, ctypes.POINTER(ctypes.c_float)]),
     "sp_tune_space": (I, [ctypes.POINTER(sp_coo), I, ctypes.POINTER(sp_tune_point), I, ctypes.POINTER(ctypes.c_int)]),
     "sp_tune_run": (I, [P, ctypes.POINTER(sp_coo), ctypes.POINTER(sp_tune_point), I, I, U64]),
+    "sp_tune_run_spmm": (I, [P, ctypes.POINTER(sp_coo), ctypes.POINTER(sp_tune_point), I, I, U64, I]),
     "sp_tune_best": (I, [ctypes.POINTER(sp_tune_point), I, I]),
     "sp_tune_db_append": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(sp_tune_point), I]),
     "sp_batch_create": (I, [P, I, PP, ctypes.POINTER(ctypes.c_int), I, PP]),

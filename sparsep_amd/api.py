@@ -579,10 +579,14 @@ def tune_space(A, kernel_id=-1):
     return pts
 
 
-def tune(ctx, A, pts=None, iters=5, flush_bytes=0):
-    """Build and time every point; returns the point array (ms, status filled)."""
+def tune(ctx, A, pts=None, iters=5, flush_bytes=0, k=1):
+    """Build and time every point (SpMV, or SpMM with k right-hand sides);
+    returns the point array (ms, status filled)."""
     pts = pts if pts is not None else tune_space(A)
-    check(lib().sp_tune_run(ctx.ptr, A.ptr, pts, len(pts), iters, flush_bytes), "sp_tune_run")
+    if k == 1:
+        check(lib().sp_tune_run(ctx.ptr, A.ptr, pts, len(pts), iters, flush_bytes), "sp_tune_run")
+    else:
+        check(lib().sp_tune_run_spmm(ctx.ptr, A.ptr, pts, len(pts), iters, flush_bytes, k), "sp_tune_run_spmm")
     return pts
 
 

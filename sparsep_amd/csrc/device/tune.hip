@@ -121,23 +121,23 @@ extern "C" int sp_tune_space(const sp_coo *A, int kernel_id, sp_tune_point *pts,
 
 // Builds run on host threads (plan creation is host work plus synchronous
 // uploads, safe to overlap), up to `kBatch` plans at a time; the timed SpMVs
-// then run one plan after another on the context stream, so timings never
-// overlap.
-extern "C" int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters,
-                           uint64_t flush_bytes) {
-    if (!ctx || !A || !pts || n < 0 || iters <= 0) return SP_ERR_ARG;
+// (k = 1) or SpMMs with k right-hand sides then run one plan after another on
+// the context stream, so timings never overlap.
+static int tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters, uint64_t flush_bytes,
+                    int k) {
+    if (!ctx || !A || !pts || n < 0 || iters <= 0 || k <= 0) return SP_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     const size_t sv = sp_dtype_size(A->dtype);
     void *x = nullptr, *y = nullptr;
-    if (hipMalloc(&x, (size_t)std::max<uint32_t>(A->ncols, 1) * sv) != hipSuccess) return SP_ERR_NOMEM;
-    if (hipMalloc(&y, (size_t)std::max<uint32_t>(A->nrows, 1) * sv) != hipSuccess) { (void)hipFree(x); return SP_ERR_NOMEM; }
+    if (hipMalloc(&x, (size_t)std::max<uint32_t>(A->ncols, 1) * k * sv) != hipSuccess) return SP_ERR_NOMEM;
+    if (hipMalloc(&y, (size_t)std::max<uint32_t>(A->nrows, 1) * k * sv) != hipSuccess) { (void)hipFree(x); return SP_ERR_NOMEM; }
     {
-        std::vector<char> hx((size_t)std::max<uint32_t>(A->ncols, 1) * sv);
+        std::vector<char> hx((size_t)std::max<uint32_t>(A->ncols, 1) * k * sv);
         sp_dist d;
         d.kind = SP_DIST_UNIFORM;
         d.lo = -1;
         d.hi = 1;
-        sp_gen_vector(A->dtype, A->ncols, d, 12345, hx.data());
+        sp_gen_vector(A->dtype, (uint64_t)A->ncols * k, d, 12345, hx.data());
         (void)hipMemcpy(x, hx.data(), hx.size(), hipMemcpyHostToDevice);
     }
     constexpr int kBatch = 16;
@@ -170,8 +170,13 @@ extern "C" int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int
                 pt.ms = -1.f;
                 continue;
             }
-            rc = sp_time_spmv(p, x, y, 1, flush_bytes, ms.data());  // warm-up
-            if (!rc) rc = sp_time_spmv(p, x, y, iters, flush_bytes, ms.data());
+            if (k == 1) {
+                rc = sp_time_spmv(p, x, y, 1, flush_bytes, ms.data());  // warm-up
+                if (!rc) rc = sp_time_spmv(p, x, y, iters, flush_bytes, ms.data());
+            } else {
+                rc = sp_time_spmm(p, k, x, k, y, k, 1, flush_bytes, ms.data());
+                if (!rc) rc = sp_time_spmm(p, k, x, k, y, k, iters, flush_bytes, ms.data());
+            }
             if (!rc) {
                 std::vector<float> s(ms);
                 std::sort(s.begin(), s.end());
@@ -190,6 +195,16 @@ extern "C" int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int
     (void)hipFree(x);
     (void)hipFree(y);
     return SP_OK;
+}
+
+extern "C" int sp_tune_run(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters,
+                           uint64_t flush_bytes) {
+    return tune_run(ctx, A, pts, n, iters, flush_bytes, 1);
+}
+
+extern "C" int sp_tune_run_spmm(sp_ctx *ctx, const sp_coo *A, sp_tune_point *pts, int n, int iters,
+                                uint64_t flush_bytes, int k) {
+    return tune_run(ctx, A, pts, n, iters, flush_bytes, k);
 }
 
 extern "C" int sp_tune_best(const sp_tune_point *pts, int n, int kernel_id) {

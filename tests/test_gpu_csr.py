@@ -177,3 +177,18 @@ def test_rmat_full_size_hot(sp, ctx):
     x = sp.gen_vector(sp.F32, A.ncols, seed=22)
     y, _ = run(sp, ctx, "CSR-nnz", A, x, sp.opts(algo=4))
     assert_parity(sp.F32, y, A, x, "rmat-full-hot")
+
+
+def test_mem_cap_reports_nomem(sp, ctx):
+    """sp_opts.mem_cap_bytes, the GPU counterpart of the reference's MRAM capacity
+    assert (spmv/1D/CSR-row/host/app.c:214-215): a plan that needs more device
+    memory than the cap fails with SP_ERR_NOMEM instead of aborting; within the
+    cap it builds, and its device bytes stay under the cap."""
+    A = sp.gen_rmat(12, 40000, dtype=4, seed=5)
+    for kern in ("CSR-nnz", "COO-nnz", "BCSR-block", "RBDCSR"):
+        with pytest.raises(sp._lib.SparsePError) as e:
+            sp.Plan(ctx, kern, A, sp.opts(mem_cap_bytes=4096))
+        assert e.value.rc == -2, kern
+        cap = 64 << 20
+        p = sp.Plan(ctx, kern, A, sp.opts(mem_cap_bytes=cap))
+        assert 0 < p.info()["device_bytes"] <= cap, kern
