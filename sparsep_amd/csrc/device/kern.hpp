@@ -649,13 +649,37 @@ __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ p
     const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
-    for (uint32_t r = r0 + wid; r < r1; r += NW) {
-        const uint32_t b = rp[r], e = rp[r + 1];
+    // This wave's rows r0 + wid + NW t: their entry ranges in one load (64 rows a
+    // round), and the next row's first 64 entries loaded while a row runs, so a
+    // row starts with its indices and values at hand.
+    for (uint32_t R0 = r0 + wid; R0 < r1; R0 += NW * WAVE) {
+    const uint32_t nrw = min((uint32_t)WAVE, (r1 - R0 + NW - 1) / NW);
+    const uint32_t Il = R0 + NW * (uint32_t)lane;
+    const uint32_t bl = (uint32_t)lane < nrw ? rp[Il] : 0u, el = (uint32_t)lane < nrw ? rp[Il + 1] : 0u;
+    auto first = [&](uint32_t t, uint32_t &cn, T &vn) {  // row t's first batch, one entry per lane
+        const uint32_t bb = (uint32_t)__builtin_amdgcn_readlane((int)bl, (int)t);
+        const uint32_t ee = (uint32_t)__builtin_amdgcn_readlane((int)el, (int)t);
+        const uint32_t nn = min((uint32_t)WAVE, ee - bb);
+        cn = (uint32_t)lane < nn ? ldg(colind + bb + lane) : 0u;
+        vn = (uint32_t)lane < nn ? ldg(vals + bb + lane) : T(0);
+    };
+    uint32_t cn;
+    T vn;
+    first(0, cn, vn);
+    for (uint32_t t = 0; t < nrw; t++) {
+        const uint32_t r = R0 + NW * t;
+        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)bl, (int)t);
+        const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)el, (int)t);
+        uint32_t cl = cn;
+        T vl = vn;
+        if (t + 1 < nrw) first(t + 1, cn, vn);
         A acc = A(0);
         for (uint32_t base = b; base < e; base += WAVE) {
             const uint32_t nb = min((uint32_t)WAVE, e - base);
-            const uint32_t cl = (uint32_t)lane < nb ? ldg(colind + base + lane) : 0u;
-            const T vl = (uint32_t)lane < nb ? ldg(vals + base + lane) : T(0);
+            if (base != b) {  // rows of more than 64 entries
+                cl = (uint32_t)lane < nb ? ldg(colind + base + lane) : 0u;
+                vl = (uint32_t)lane < nb ? ldg(vals + base + lane) : T(0);
+            }
             for (uint32_t t0 = 0; t0 < nb; t0 += G * U) {
                 T v[U], xv[U];
 #pragma unroll
@@ -675,6 +699,7 @@ __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ p
         }
         acc = xor_sum<K, WAVE>(acc);
         if (g == 0) y[(size_t)r * ldy + q] = to_val<T>(acc);
+    }
     }
     }
 }
@@ -1211,8 +1236,25 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
     const uint32_t ncols = P.ncols_elem;
-    for (uint32_t I = r0 + wid; I < r1; I += NW) {
-        const uint32_t b = rp[I], e = rp[I + 1];
+    // This wave's block rows r0 + wid + NW t: their block ranges in one load (64 rows
+    // a round), and the next row's first block columns loaded while a row runs.
+    constexpr uint32_t NBATCH = sizeof(T) == 4 ? WAVE : NB64;  // blocks per batch
+    for (uint32_t R0 = r0 + wid; R0 < r1; R0 += NW * WAVE) {
+    const uint32_t nrw = min((uint32_t)WAVE, (r1 - R0 + NW - 1) / NW);
+    const uint32_t Il = R0 + NW * (uint32_t)lane;
+    const uint32_t bl = (uint32_t)lane < nrw ? rp[Il] : 0u, el = (uint32_t)lane < nrw ? rp[Il + 1] : 0u;
+    auto first_cols = [&](uint32_t t) {  // row t's first batch of block columns, one per lane
+        const uint32_t bb = (uint32_t)__builtin_amdgcn_readlane((int)bl, (int)t);
+        const uint32_t ee = (uint32_t)__builtin_amdgcn_readlane((int)el, (int)t);
+        return (uint32_t)lane < min(NBATCH, ee - bb) ? ldg(cip + bb + lane) : 0u;
+    };
+    uint32_t Jn = first_cols(0);
+    for (uint32_t t = 0; t < nrw; t++) {
+        const uint32_t I = R0 + NW * t;
+        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)bl, (int)t);
+        const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)el, (int)t);
+        const uint32_t Jfirst = Jn;
+        if (t + 1 < nrw) Jn = first_cols(t + 1);
         if constexpr (sizeof(T) == 4) {
             constexpr int G = WAVE / K;  // matrix blocks per step
             constexpr int U = 8 / G;     // steps whose loads go out together
@@ -1222,7 +1264,7 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
             // handed to the lanes that need them by a shuffle
             for (uint32_t base = b; base < e; base += WAVE) {
                 const uint32_t nb = min((uint32_t)WAVE, e - base);
-                const uint32_t Jl = (uint32_t)lane < nb ? ldg(cip + base + lane) : 0u;
+                const uint32_t Jl = base == b ? Jfirst : ((uint32_t)lane < nb ? ldg(cip + base + lane) : 0u);
                 for (uint32_t t0 = 0; t0 < nb; t0 += G * U) {
                     T a[U][4], xb[U][4];
                     bool act[U];
@@ -1279,7 +1321,7 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
                 __builtin_amdgcn_wave_barrier();
                 for (uint32_t c = lane; c < nb * 8; c += WAVE)
                     *(u32x4 *)(sv + c * 2) = __builtin_nontemporal_load((gptr<const u32x4>)(bv + (size_t)base * 16) + c);
-                const uint32_t Jl = (uint32_t)lane < nb ? ldg(cip + base + lane) : 0u;
+                const uint32_t Jl = base == b ? Jfirst : ((uint32_t)lane < nb ? ldg(cip + base + lane) : 0u);
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 for (uint32_t t0 = 0; t0 < nb; t0 += BPS * U) {
@@ -1324,6 +1366,7 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
                 }
             }
         }
+    }
     }
     }
 }
