@@ -612,6 +612,12 @@ int sp_cg(sp_plan *A, const void *b, void *x, int maxit, double rtol, int *iters
  * ms per copy; the measured stream-copy peak is 2*bytes/ms. */
 int sp_time_copy(sp_ctx *ctx, uint64_t bytes, int iters, float *ms);
 
+/* Bandwidth probe: time a read-only stream over `bytes` (16-byte
+ * non-temporal loads, nothing written), returns ms per pass; the measured
+ * read peak is bytes/ms.  SpMV streams are almost all reads, so this is the
+ * ceiling the SpMV kernels are compared with beside the copy. */
+int sp_time_read(sp_ctx *ctx, uint64_t bytes, int iters, float *ms);
+
 /* ---------------------------------------------------------------------
  * Learned variant selector (no counterpart in the reference; its only
  * precursor is the block fill ratio computed in

@@ -165,6 +165,7 @@ SIGNATURES = {
     "sp_time_spmv_split": (I, [P, P, P, I, U64, ctypes.POINTER(ctypes.c_float),
                                ctypes.POINTER(ctypes.c_float)]),
     "sp_time_copy": (I, [P, U64, I, ctypes.POINTER(ctypes.c_float)]),
+    "sp_time_read": (I, [P, U64, I, ctypes.POINTER(ctypes.c_float)]),
     "sp_tune_space": (I, [ctypes.POINTER(sp_coo), I, ctypes.POINTER(sp_tune_point), I, ctypes.POINTER(ctypes.c_int)]),
     "sp_tune_run": (I, [P, ctypes.POINTER(sp_coo), ctypes.POINTER(sp_tune_point), I, I, U64]),
     "sp_tune_run_spmm": (I, [P, ctypes.POINTER(sp_coo), ctypes.POINTER(sp_tune_point), I, I, U64, I]),

@@ -398,6 +398,12 @@ class Context:
         check(lib().sp_time_copy(self.ptr, nbytes, iters, ms), "sp_time_copy")
         return list(ms)
 
+    def time_read(self, nbytes, iters=10):
+        """ms per read-only pass over nbytes of device memory (sp_time_read)."""
+        ms = (ctypes.c_float * iters)()
+        check(lib().sp_time_read(self.ptr, nbytes, iters, ms), "sp_time_read")
+        return list(ms)
+
 
 def gpu_present():
     """True when a GPU kernel driver node exists (does not initialise HIP)."""

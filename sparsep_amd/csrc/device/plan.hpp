@@ -77,6 +77,7 @@ int run_plan(sp_plan *p, const void *x, void *y, int k, int ldx, int ldy);
 int launch_fill_zero(hipStream_t s, void *ptr, uint64_t bytes);
 int launch_flush(hipStream_t s, void *buf, uint64_t bytes, uint32_t salt);
 int launch_copy(hipStream_t s, const void *src, void *dst, uint64_t bytes);
+int launch_read(hipStream_t s, const void *src, uint64_t bytes, uint32_t *sink);
 int launch_reduce_parts(hipStream_t s, int dtype, const void *ypart, uint32_t vert, uint64_t stride,
                         void *y, uint64_t n, int k, int ldy);
 

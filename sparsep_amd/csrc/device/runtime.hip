@@ -409,3 +409,28 @@ extern "C" int sp_time_copy(sp_ctx *c, uint64_t bytes, int iters, float *ms) {
     hipFree(b);
     return rc;
 }
+
+extern "C" int sp_time_read(sp_ctx *c, uint64_t bytes, int iters, float *ms) {
+    if (!c || !ms || iters <= 0 || bytes < 16) return SP_ERR_ARG;
+    hipSetDevice(c->device);
+    void *a = nullptr, *sink = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return SP_ERR_NOMEM;
+    if (hipMalloc(&sink, 256) != hipSuccess) { hipFree(a); return SP_ERR_NOMEM; }
+    int rc = launch_fill_zero(c->stream, a, bytes);
+    for (int it = 0; it < iters && !rc; it++) {
+        if (hipEventRecord(c->ev0, c->stream) != hipSuccess) { rc = SP_ERR_HIP; break; }
+        rc = launch_read(c->stream, a, bytes, (uint32_t *)sink);
+        if (rc) break;
+        if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess) {
+            rc = SP_ERR_HIP;
+            break;
+        }
+        float t = 0.f;
+        hipEventElapsedTime(&t, c->ev0, c->ev1);
+        ms[it] = t;
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(a);
+    hipFree(sink);
+    return rc;
+}

@@ -31,6 +31,27 @@ __global__ void copy_kernel(const u32x4 *__restrict__ src, u32x4 *__restrict__ d
         dst[i] = __builtin_nontemporal_load(src + i);
 }
 
+// Read-only stream: every 16-byte word of src is loaded once (non-temporal,
+// four loads in flight per thread) and folded by XOR into one word per
+// thread.  The word is stored only when it equals a value that the
+// zero-filled buffer cannot produce, so the loads stay live and the kernel
+// writes nothing.
+__global__ void read_kernel(const u32x4 *__restrict__ src, uint64_t n16, uint32_t *__restrict__ sink) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    u32x4 acc = {0u, 0u, 0u, 0u};
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const u32x4 a = __builtin_nontemporal_load(src + i);
+        const u32x4 b = __builtin_nontemporal_load(src + i + stride);
+        const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+        const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+        acc ^= a ^ b ^ c ^ d;
+    }
+    for (; i < n16; i += stride) acc ^= __builtin_nontemporal_load(src + i);
+    const uint32_t v = acc.x ^ acc.y ^ acc.z ^ acc.w;
+    if (v == 0x9E3779B9u) sink[0] = v;
+}
+
 // y[i] = sum over v of ypart[v*stride + i] in part order (the host merge of
 // spmv/2D/DCSR/host/app.c:340-351 and its RBD/BD twins).
 template <typename T>
@@ -84,6 +105,15 @@ int launch_copy(hipStream_t s, const void *src, void *dst, uint64_t bytes) {
     if (!n16) return SP_OK;
     hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n16, 256)), dim3(256), 0, s, (const u32x4 *)src,
                        (u32x4 *)dst, n16);
+    return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
+}
+
+int launch_read(hipStream_t s, const void *src, uint64_t bytes, uint32_t *sink) {
+    const uint64_t n16 = bytes / 16;
+    if (!n16) return SP_OK;
+    uint64_t g = (n16 + 4 * 256 - 1) / (4 * 256);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(read_kernel, dim3((unsigned)g), dim3(256), 0, s, (const u32x4 *)src, n16, sink);
     return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
 }
 

@@ -192,3 +192,17 @@ def test_mem_cap_reports_nomem(sp, ctx):
         cap = 64 << 20
         p = sp.Plan(ctx, kern, A, sp.opts(mem_cap_bytes=cap))
         assert 0 < p.info()["device_bytes"] <= cap, kern
+
+
+def test_bandwidth_probes(sp, ctx):
+    """sp_time_copy and sp_time_read time every pass and give rates below the
+    published 8 TB/s of HBM; bad arguments are refused."""
+    import ctypes
+    n = 256 << 20
+    for name, fn, moved in (("copy", ctx.time_copy, 2 * n), ("read", ctx.time_read, n)):
+        ms = fn(n, 4)
+        assert len(ms) == 4 and all(t > 0 for t in ms), name
+        assert moved / (min(ms) * 1e-3) < 8.5e12, name
+    out = (ctypes.c_float * 1)()
+    assert sp.lib().sp_time_read(ctx.ptr, 8, 1, out) == -1  # SP_ERR_ARG
+    assert sp.lib().sp_time_read(ctx.ptr, n, 0, out) == -1
