@@ -24,6 +24,8 @@
 // All reductions have a fixed order, so results are run-to-run identical.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace spd {
@@ -1368,6 +1370,132 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
         }
     }
     }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BCSR 4 x 4 FP32 blocks times 32 right-hand sides on the vector ALUs (the
+// block-row product of spmv/1D/BCSR-nnz/dpu/task.c:337-370 over a 4 x 32
+// panel of x per block).  A wave takes a block row; it stages the values of
+// its blocks in LDS 64 blocks at a time (coalesced 16-byte loads) and hands
+// their block columns to the lanes by shuffles.  Lane l = 8 g + r takes
+// block g of the step (8 blocks per step) and right-hand sides 4r .. 4r + 3:
+// it reads the block's 16 values from LDS (the 8 lanes of a block read the
+// same words) and the four x rows' 16-byte pieces it needs, and keeps a
+// 4 x 4 tile of sums, added over the 8 block slots at the end of the row.
+// Measured on the 27-point stencil (64^3 x 4): 409-417 us against 511-515 us
+// for the matrix-core kernel, whose 4-byte x loads (one column per lane, as
+// the 4 x 4 x 1 instruction wants its B operand) issue four times as many
+// load instructions; at 8 columns the matrix-core kernel stays faster
+// (207-213 against 250-264 us).
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(TPB) void bcsr4_valu_kernel(const Prob *__restrict__ probs,
+                                                         const uint32_t *__restrict__ unit_pid,
+                                                         const uint32_t *__restrict__ urow, Ptrs<float> io,
+                                                         uint32_t nunits) {
+    static_assert(K == 32, "32 right-hand sides per pass");
+    constexpr int NW = TPB / WAVE;
+    constexpr int GS = K / 4;     // lanes per block
+    constexpr int G = WAVE / GS;  // blocks per step
+    __shared__ __align__(16) float s_val[NW][WAVE * 16];  // a wave's batch of 64 blocks
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    const int g = lane / GS, r4 = lane % GS;
+    float *const sv = s_val[wid];
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint32_t pid = unit_pid[u];
+    const Prob P = probs[pid];
+    const gptr<const float> __restrict__ x = gp(io.x(pid, P));
+    const gptr<float> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), cip = gp(P.cind);
+    const gptr<const float> __restrict__ bv = gp((const float *)P.val);
+    const int ldx = io.lx(P), ldy = io.ly(P);
+    const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    const uint32_t ncols = P.ncols_elem;
+    // whole 16-byte pieces of x and y rows (uniform over the launch); the x case is a
+    // template argument so that no load sits under a branch
+    const bool xw = ((uintptr_t)io.x(pid, P) % 16 == 0) && (ldx % 4 == 0);
+    const bool yw = ((uintptr_t)io.y(pid, P) % 16 == 0) && (ldy % 4 == 0);
+    auto rows = [&](auto xw_c) {
+        constexpr bool XW = decltype(xw_c)::value;
+        for (uint32_t I = r0 + wid; I < r1; I += NW) {
+            const uint32_t b = rp[I], e = rp[I + 1];
+            float acc[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[i][q] = 0.f;
+            for (uint32_t base = b; base < e; base += WAVE) {
+                const uint32_t nb = min((uint32_t)WAVE, e - base);
+                // the batch's values to LDS in 16-byte pieces, its block columns to lanes
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int c0 = 0; c0 < 4; c0++) {
+                    const uint32_t c = (uint32_t)lane + (uint32_t)c0 * WAVE;  // piece c of the batch
+                    if (c < nb * 4)
+                        *(u32x4 *)(sv + c * 4) = __builtin_nontemporal_load((gptr<const u32x4>)(bv + (size_t)base * 16) + c);
+                }
+                const uint32_t Jl = (uint32_t)lane < nb ? ldg(cip + base + lane) : 0u;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t t0 = 0; t0 < nb; t0 += G) {
+                    const uint32_t sl = t0 + g;
+                    const bool act = sl < nb;
+                    const uint32_t sc = act ? sl : 0u;  // a valid block; its products are zeroed
+                    const uint32_t J = (uint32_t)__shfl((int)Jl, (int)sc, WAVE);
+                    float a[16], xr[4][4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const f32x4 v = *(const f32x4 *)(sv + sc * 16 + i * 4);
+#pragma unroll
+                        for (int c = 0; c < 4; c++) a[i * 4 + c] = act ? v[c] : 0.f;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const uint32_t col = J * 4 + c;
+                        const bool ok = act && col < ncols;
+                        const gptr<const float> px = x + (size_t)(ok ? col : 0u) * ldx + 4 * r4;
+                        if constexpr (XW) {
+                            const f32x4 v = *(gptr<const f32x4>)px;
+#pragma unroll
+                            for (int q = 0; q < 4; q++) xr[c][q] = ok ? v[q] : 0.f;
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < 4; q++) xr[c][q] = ok ? px[q] : 0.f;
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+#pragma unroll
+                            for (int q = 0; q < 4; q++) acc[i][q] = __builtin_fmaf(a[i * 4 + c], xr[c][q], acc[i][q]);
+                }
+                __builtin_amdgcn_wave_barrier();  // the next batch overwrites the staged values
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[i][q] = xor_sum<GS, WAVE>(acc[i][q]);
+            if (g == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t row = I * 4 + i;
+                    if (row >= P.nrows_elem) continue;
+                    const gptr<float> py = y + (size_t)row * ldy + 4 * r4;
+                    if (yw) {
+                        const f32x4 v = {acc[i][0], acc[i][1], acc[i][2], acc[i][3]};
+                        *(gptr<f32x4>)py = v;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; q++) py[q] = acc[i][q];
+                    }
+                }
+            }
+        }
+    };
+    if (xw) rows(std::true_type{});
+    else rows(std::false_type{});
     }
 }
 

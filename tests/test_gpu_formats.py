@@ -256,12 +256,14 @@ def test_coo_strided_output(sp, ctx, dt):
 
 
 @pytest.mark.parametrize("dt", [4, 5])
-@pytest.mark.parametrize("k", [8, 16, 24, 32, 40, 64])
+@pytest.mark.parametrize("k", [8, 16, 24, 32, 35, 40, 64])
 def test_bcsr4_spmm_matrix_cores(sp, ctx, dt, k):
-    """4x4 BCSR blocks times 8/16/32-column chunks run on the matrix cores:
-    dense 4x4 blocks (4 unknowns per grid point), a truncated last block row
-    and block column (rows and columns not multiples of 4), empty block rows,
-    1D plans of both balances, 2D tilings, and every chunk size."""
+    """4x4 BCSR blocks times 8/16/32-column chunks (matrix cores; FP32
+    32-column chunks on the vector ALUs): dense 4x4 blocks (4 unknowns per
+    grid point), a truncated last block row and block column (rows and
+    columns not multiples of 4), empty block rows, 1D plans of both
+    balances, 2D tilings, every chunk size, and x / y rows of 35 values
+    (not whole 16-byte pieces)."""
     cases = [sp.gen_stencil27(7, 4, True, dt, seed=7),
              sp.gen_uniform(1001, 998, 0, 30, dtype=dt, seed=8)]
     for A in cases:
