@@ -619,18 +619,20 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
 
 // ---------------------------------------------------------------------------
 // CSR, row granular, K >= 8 right-hand sides: lanes across the right-hand
-// sides.  A wave takes a row; lane l = K*g + q works on column q of entry
-// slot g (G = 64 / K entries of the row per step).  The row's column
-// indices and values are read 64 at a time, one entry per lane, and handed
-// out by shuffles, so each step is one coalesced read of G rows of x (K
-// consecutive values each) and one multiply-add per lane; the G partial
-// sums are added across lanes at the end of the row.  The row loop of the
-// reference tasklet (spmv/1D/CSR-row/dpu/task.c:105-202) with the
-// right-hand sides spread over lanes instead of entries, so no per-column
-// reduction over many lanes is needed.  Used where a row of x is 64 bytes or
-// more (K * sizeof(T) >= 64); narrower rows keep csr_vector_kernel, whose
-// lanes read whole x rows in 16-byte pieces (FP32, 8 columns on the 27-point
-// stencil: 532 against 667 us).
+// sides.  A wave takes a row; lane l = GS*g + r works on columns V r ..
+// V r + V - 1 (one 16-byte piece of an x row, V = 16 / sizeof(T), GS = K / V
+// lanes per entry) of entry slot g (G = 64 / GS entries of the row per
+// step).  The row's column indices and values are read 64 at a time, one
+// entry per lane, and handed out by shuffles, so each step is one coalesced
+// read of G rows of x and V multiply-adds per lane; the G partial sums are
+// added across lanes at the end of the row.  The row loop of the reference
+// tasklet (spmv/1D/CSR-row/dpu/task.c:105-202) with the right-hand sides
+// spread over lanes instead of entries, so no per-column reduction over many
+// lanes is needed.  (One column per lane, 4-byte x loads, took twice as long:
+// 27-point stencil, FP32, 32 columns: 1648 against 832 us.)  Used where a row
+// of x is 64 bytes or more (K * sizeof(T) >= 64); narrower rows keep
+// csr_vector_kernel, whose lanes read whole x rows in 16-byte pieces (FP32,
+// 8 columns on the 27-point stencil: 532 against 667 us).
 // ---------------------------------------------------------------------------
 template <typename T, int K>
 __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ probs,
@@ -639,9 +641,12 @@ __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ p
                                                       uint32_t nunits) {
     static_assert(K == 8 || K == 16 || K == 32, "8, 16 or 32 right-hand sides per pass");
     using A = acc_t<T>;
-    constexpr int NW = TPB / WAVE, G = WAVE / K, U = 4;
+    constexpr int V = 16 / (int)sizeof(T);  // columns per lane: one 16-byte piece of an x row
+    static_assert(K % V == 0, "whole 16-byte pieces per x row");
+    constexpr int GS = K / V;               // lanes per entry
+    constexpr int NW = TPB / WAVE, G = WAVE / GS, U = 4;
     const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
-    const int g = lane / K, q = lane % K;
+    const int g = lane / GS, r4 = lane % GS;
     for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const uint32_t pid = unit_pid[u];
     const Prob P = probs[pid];
@@ -651,6 +656,12 @@ __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ p
     const gptr<const T> __restrict__ vals = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    // whole 16-byte pieces of x and y rows (uniform over the launch); the x case is a
+    // template argument so that no load sits under a branch
+    const bool xw = ((uintptr_t)io.x(pid, P) % 16 == 0) && (ldx % V == 0);
+    const bool yw = ((uintptr_t)io.y(pid, P) % 16 == 0) && (ldy % V == 0);
+    auto rows = [&](auto xw_c) {
+    constexpr bool XW = decltype(xw_c)::value;
     // This wave's rows r0 + wid + NW t: their entry ranges in one load (64 rows a
     // round), and the next row's first 64 entries loaded while a row runs, so a
     // row starts with its indices and values at hand.
@@ -675,7 +686,9 @@ __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ p
         uint32_t cl = cn;
         T vl = vn;
         if (t + 1 < nrw) first(t + 1, cn, vn);
-        A acc = A(0);
+        A acc[V];
+#pragma unroll
+        for (int j = 0; j < V; j++) acc[j] = A(0);
         for (uint32_t base = b; base < e; base += WAVE) {
             const uint32_t nb = min((uint32_t)WAVE, e - base);
             if (base != b) {  // rows of more than 64 entries
@@ -683,26 +696,53 @@ __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ p
                 vl = (uint32_t)lane < nb ? ldg(vals + base + lane) : T(0);
             }
             for (uint32_t t0 = 0; t0 < nb; t0 += G * U) {
-                T v[U], xv[U];
+                T v[U], xv[U][V];
 #pragma unroll
                 for (int w = 0; w < U; w++) {
                     const uint32_t s = t0 + w * G + g;
                     const bool act = s < nb;
+                    // every lane takes part in the shuffles (a lane reads no value from an
+                    // inactive one); an idle slot reads entry 0's x row and adds 0 times it
                     const int src = act ? (int)s : 0;
-                    // every lane takes part in the shuffles (a lane reads no value from an inactive one)
                     const uint32_t c = (uint32_t)__shfl((int)cl, src, WAVE);
                     const T vs = (T)__shfl(vl, src, WAVE);
                     v[w] = act ? vs : T(0);
-                    xv[w] = act ? x[(size_t)c * ldx + q] : T(0);
+                    const gptr<const T> px = x + (size_t)c * ldx + V * r4;
+                    if constexpr (XW) {
+                        const u32x4 p4 = *(gptr<const u32x4>)px;
+                        __builtin_memcpy(xv[w], &p4, 16);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < V; j++) xv[w][j] = px[j];
+                    }
                 }
 #pragma unroll
-                for (int w = 0; w < U; w++) acc = fma_acc<T>(v[w], xv[w], acc);
+                for (int w = 0; w < U; w++)
+#pragma unroll
+                    for (int j = 0; j < V; j++) acc[j] = fma_acc<T>(v[w], xv[w][j], acc[j]);
             }
         }
-        acc = xor_sum<K, WAVE>(acc);
-        if (g == 0) y[(size_t)r * ldy + q] = to_val<T>(acc);
+#pragma unroll
+        for (int j = 0; j < V; j++) acc[j] = xor_sum<GS, WAVE>(acc[j]);
+        if (g == 0) {
+            const gptr<T> py = y + (size_t)r * ldy + V * r4;
+            T o[V];
+#pragma unroll
+            for (int j = 0; j < V; j++) o[j] = to_val<T>(acc[j]);
+            if (yw) {
+                u32x4 p4;
+                __builtin_memcpy(&p4, o, 16);
+                *(gptr<u32x4>)py = p4;
+            } else {
+#pragma unroll
+                for (int j = 0; j < V; j++) py[j] = o[j];
+            }
+        }
     }
     }
+    };
+    if (xw) rows(std::true_type{});
+    else rows(std::false_type{});
     }
 }
 
