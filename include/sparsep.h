@@ -400,7 +400,10 @@ const char *sp_kernel_name(int id);
 /* GPU algorithm inside a variant. */
 typedef enum sp_algo {
     SP_ALGO_AUTO = 0,
-    SP_ALGO_VECTOR = 1,    /* row granular: `lanes_per_row` lanes per row   */
+    /* row granular: `lanes_per_row` lanes per row (1D CSR with
+     * items_per_lane 4: lanes of 4-entry chunks; AUTO takes that for even
+     * rows of 16 entries or more) */
+    SP_ALGO_VECTOR = 1,
     /* entry balanced: fixed entry tiles, rows split.  BCSR ids 5 and 6:
      * merge-path tiles over block-row ends and blocks (AUTO takes them when
      * the longest block row exceeds 4 * average + 64 blocks). */
@@ -446,7 +449,9 @@ typedef struct sp_opts {
     int32_t sub_balance;      /* sp_balance for the sub-part split              */
     int32_t algo;             /* sp_algo                                        */
     uint32_t lanes_per_row;   /* VECTOR: 1,2,4,...,64; 0 = auto                 */
-    uint32_t items_per_lane;  /* SEGMENTED: entries per lane per tile; 0 = auto */
+    uint32_t items_per_lane;  /* SEGMENTED: entries per lane per tile; 0 = auto; *
+                               * VECTOR (1D CSR): 4 = each lane loads 4-entry   *
+                               * chunks (lanes_per_row lanes of chunks)         */
     uint32_t vert;            /* 2D vertical partitions (-v); 0 = auto          */
     uint32_t row_block_size;  /* B*: block rows (-r), default 4                 */
     uint32_t col_block_size;  /* B*: block cols (-c), default 4                 */

@@ -618,6 +618,82 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// CSR, row granular, one right-hand side, 4-entry chunks: L lanes per row,
+// each lane taking whole 4-entry chunks aligned in the arrays (one 16-byte
+// index load and one 4 * sizeof(T) value load per chunk; the plan's arrays
+// are aligned to 4 entries and readable 3 entries past their end).  A row's
+// chunks start at its first entry rounded down to a multiple of 4; entries
+// of a chunk outside the row add nothing and gather x[0].  Two chunks per
+// lane are in flight in the main loop; no load sits under a branch.  The
+// row loop of spmv/1D/CSR-row/dpu/task.c:105-202 with four entries per lane
+// and load.  (27-point stencil 64^3 x 4, FP32: 245 us with 32 lanes against
+// 301-363 us for one entry per lane and 303 us for the merge-path kernel.)
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+__global__ __launch_bounds__(TPB) void csr_chunk_kernel(const Prob *__restrict__ probs,
+                                                        const uint32_t *__restrict__ unit_pid,
+                                                        const uint32_t *__restrict__ urow, Ptrs<T> io,
+                                                        uint32_t nunits) {
+    using A = acc_t<T>;
+    constexpr int G = TPB / L;
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const uint32_t pid = unit_pid[u];
+    const Prob P = probs[pid];
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), cip = gp(P.cind);
+    const gptr<const T> __restrict__ vals = gp((const T *)P.val);
+    const int ldx = io.lx(P), ldy = io.ly(P);
+    const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
+    const int lane = threadIdx.x % L, grp = threadIdx.x / L;
+    // chunk at k: its entries' products, those outside [b, e) zero
+    auto chunk = [&](uint32_t k, uint32_t b, uint32_t e, A acc) {
+        uint32_t c[4];
+        T v[4], xv[4];
+        load4<T>(cip, vals, k, c, v);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const bool ok = k + i >= b && k + i < e;
+            xv[i] = x[(size_t)(ok ? c[i] : 0u) * ldx];
+            if (!ok) v[i] = T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc = fma_acc<T>(v[i], xv[i], acc);
+        return acc;
+    };
+    for (uint32_t r = r0 + grp; r < r1; r += G) {
+        const uint32_t b = rp[r], e = rp[r + 1];
+        A acc = A(0);
+        if (e > b) {
+            uint32_t k = (b & ~3u) + 4 * (uint32_t)lane;
+            for (; k + 4 * L < e; k += 8 * L) {  // two chunks per lane in flight
+                uint32_t c[2][4];
+                T v[2][4], xv[2][4];
+                load4<T>(cip, vals, k, c[0], v[0]);
+                load4<T>(cip, vals, k + 4 * L, c[1], v[1]);
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const uint32_t kk = k + h * 4 * L + i;
+                        const bool ok = kk >= b && kk < e;
+                        xv[h][i] = x[(size_t)(ok ? c[h][i] : 0u) * ldx];
+                        if (!ok) v[h][i] = T(0);
+                    }
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) acc = fma_acc<T>(v[h][i], xv[h][i], acc);
+            }
+            if (k < e) acc = chunk(k, b, e, acc);
+        }
+        acc = group_sum<L>(acc);
+        if (lane == 0) y[(size_t)r * ldy] = to_val<T>(acc);
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // CSR, row granular, K >= 8 right-hand sides: lanes across the right-hand
 // sides.  A wave takes a row; lane l = GS*g + r works on columns V r ..
 // V r + V - 1 (one 16-byte piece of an x row, V = 16 / sizeof(T), GS = K / V

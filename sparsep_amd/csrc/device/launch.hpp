@@ -120,6 +120,19 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
             hipLaunchKernelGGL((csr_rhs_kernel<T, 8>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits);
             break;
         }
+        if constexpr (K == 1) {
+            if (w.ipt == 4 && w.chunk) {  // 4-entry chunks per lane (the plan's arrays: aligned and padded)
+                switch (w.lanes) {
+                case 4: hipLaunchKernelGGL((csr_chunk_kernel<T, 4>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
+                case 8: hipLaunchKernelGGL((csr_chunk_kernel<T, 8>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
+                case 16: hipLaunchKernelGGL((csr_chunk_kernel<T, 16>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
+                case 32: hipLaunchKernelGGL((csr_chunk_kernel<T, 32>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
+                case 64: hipLaunchKernelGGL((csr_chunk_kernel<T, 64>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
+                default: return SP_ERR_ARG;
+                }
+                break;
+            }
+        }
 #define SPD_VEC(Lv) case Lv: hipLaunchKernelGGL((csr_vector_kernel<T, Lv, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
         switch (w.lanes) {
             SPD_VEC(1) SPD_VEC(2) SPD_VEC(4) SPD_VEC(8) SPD_VEC(16) SPD_VEC(32) SPD_VEC(64)

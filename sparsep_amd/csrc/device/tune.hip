@@ -41,6 +41,17 @@ static void space_for(const sp_coo *A, int id, std::vector<Pt> &v) {
             p.o.lanes_per_row = L;
             v.push_back(p);
         }
+        // the row kernel with 4-entry chunk loads, lanes around the average row's chunks
+        uint32_t Lq = 4;
+        while (Lq < 64 && 4.0 * Lq < avg) Lq *= 2;
+        for (uint32_t L : {Lq / 2, Lq, Lq * 2}) {
+            if (L < 4 || L > 64) continue;
+            Pt p{id, base};
+            p.o.algo = SP_ALGO_VECTOR;
+            p.o.lanes_per_row = L;
+            p.o.items_per_lane = 4;
+            v.push_back(p);
+        }
         for (uint32_t I : {4u, 8u, 16u}) {
             Pt p{id, base};
             p.o.algo = SP_ALGO_SEGMENTED;

@@ -16,7 +16,9 @@ ALGOS = [("vec1", dict(algo=1, lanes_per_row=1)), ("vec4", dict(algo=1, lanes_pe
          ("vec16", dict(algo=1, lanes_per_row=16)), ("vec64", dict(algo=1, lanes_per_row=64)),
          ("seg4", dict(algo=2, items_per_lane=4)), ("seg8", dict(algo=2, items_per_lane=8)),
          ("seg16", dict(algo=2, items_per_lane=16)), ("sorted4", dict(algo=3, items_per_lane=4)),
-         ("sorted16", dict(algo=3, items_per_lane=16)), ("hot", dict(algo=4))]
+         ("sorted16", dict(algo=3, items_per_lane=16)), ("hot", dict(algo=4)),
+         ("chunk4", dict(algo=1, lanes_per_row=4, items_per_lane=4)),
+         ("chunk32", dict(algo=1, lanes_per_row=32, items_per_lane=4))]
 
 
 def test_stencil5_int32_bit_exact(sp, ctx):
