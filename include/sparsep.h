@@ -450,8 +450,8 @@ typedef struct sp_opts {
     int32_t algo;             /* sp_algo                                        */
     uint32_t lanes_per_row;   /* VECTOR: 1,2,4,...,64; 0 = auto                 */
     uint32_t items_per_lane;  /* SEGMENTED: entries per lane per tile; 0 = auto; *
-                               * VECTOR (1D CSR): 4 = each lane loads 4-entry   *
-                               * chunks (lanes_per_row lanes of chunks)         */
+                               * VECTOR (1D and 2D CSR): 4 = each lane loads    *
+                               * 4-entry chunks (lanes_per_row lanes of chunks) */
     uint32_t vert;            /* 2D vertical partitions (-v); 0 = auto          */
     uint32_t row_block_size;  /* B*: block rows (-r), default 4                 */
     uint32_t col_block_size;  /* B*: block cols (-c), default 4                 */

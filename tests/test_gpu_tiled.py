@@ -32,9 +32,22 @@ def test_2d_csr_algorithms_and_blocks(sp, ctx, dt):
     A = sp.gen_rmat(11, 25000, dtype=dt, seed=9)
     x = sp.gen_vector(dt, A.ncols, seed=2)
     for kern in ("DCSR", "RBDCSR", "BDCSR"):
-        for kw in (dict(algo=1, lanes_per_row=4), dict(algo=2, items_per_lane=4), dict(algo=2, items_per_lane=16)):
+        for kw in (dict(algo=1, lanes_per_row=4), dict(algo=2, items_per_lane=4), dict(algo=2, items_per_lane=16),
+                   dict(algo=1, items_per_lane=4), dict(algo=1, items_per_lane=4, lanes_per_row=16)):
             y, _ = run(sp, ctx, kern, A, x, sp.opts(vert=4, **kw))
             assert_parity(dt, y, A, x, f"{kern}/{kw}")
+    # 2D CSR tiles in the chunked row kernel (each tile's entries 4-aligned in the arrays),
+    # also with 3 right-hand sides
+    B = sp.gen_stencil27(8, 4, True, dt, seed=6)
+    xb = sp.gen_vector(dt, B.ncols, seed=3)
+    Xb = np.stack([sp.gen_vector(dt, B.ncols, seed=30 + q) for q in range(3)], axis=1)
+    for kern in ("DCSR", "RBDCSR", "BDCSR"):
+        for kw in (dict(vert=3), dict(vert=2, nparts=8, nsub=3)):
+            y, plan = run(sp, ctx, kern, B, xb, sp.opts(algo=1, items_per_lane=4, **kw))
+            assert plan.info()["algo"] == 1 and plan.info()["items_per_lane"] == 4
+            assert_parity(dt, y, B, xb, f"stencil {kern}/{kw}")
+        Y, _ = run(sp, ctx, kern, B, Xb, sp.opts(algo=1, items_per_lane=4, vert=3), k=3)
+        assert_parity_multi(dt, Y, B, Xb, f"stencil spmm {kern}")
     for kern in ("DBCSR", "RBDBCSR-nnz", "BDBCOO-block"):
         for r, c in ((2, 2), (8, 8), (2, 4)):
             y, _ = run(sp, ctx, kern, A, x, sp.opts(vert=3, row_block_size=r, col_block_size=c))
