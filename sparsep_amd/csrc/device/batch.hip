@@ -213,6 +213,24 @@ extern "C" int sp_batch_create(sp_ctx *ctx, int nprob, sp_plan *const *plans, co
             if ((rc = balloc(b, n * 4, (void **)&w.d_carry_row))) break;
             if ((rc = balloc(b, n * g.kc * asz, &w.d_carry_val))) break;
         }
+        if (w.kind == WK_CSR_MERGE && !w.nzero && w.nunits) {
+            // the rows crossing tiles as runs (unit t carries the row at its end coordinate),
+            // as a single plan lists them: each run's carries are added with independent loads
+            // instead of a walk over the units that stops at the run's end
+            std::vector<spd::Run> runs;
+            for (uint32_t t = 0; t < w.nunits; t++) {
+                const uint32_t pid = a.unit_pid[t], r = a.c0[(size_t)t + pid + 1];
+                const spd::Prob &P = a.probs[pid];
+                if (r >= P.nrows || r >= P.nrows_elem) continue;
+                if (!runs.empty()) {
+                    spd::Run &q = runs.back();
+                    if (q.pid == pid && q.row == r && q.t0 + q.n == t) { q.n++; continue; }
+                }
+                runs.push_back(spd::Run{pid, t, 1, r, P.yoff + (int64_t)r, P.ldy, 0});
+            }
+            w.nruns = (uint32_t)runs.size();
+            if ((rc = bupload(b, runs, &w.d_runs))) break;
+        }
         g.tab_off = tab;
         tab += 2 * (uint32_t)g.slots.size();
     }
