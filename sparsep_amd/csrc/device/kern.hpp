@@ -261,25 +261,29 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH ? 6 : 1))
         const uint32_t a0 = j0 & ~3u, lead = j0 - a0, nch = (lead + nb + 3) >> 2;
         uint32_t cv[R][4];
         T vv[R][4], xv[R][4];
+        // No load under a branch (the compiler would wait for the earlier loads at each
+        // branch's end, one gather in flight at a time): chunks past the tile's last load
+        // it again, and entries outside the tile gather x[0].  (INT8 R-MAT 2^20 CSR: 68.6
+        // against 75-81 us per SpMV; for the lane-strided path below the same change was
+        // slower, 86 against 80 us FP32, and is not made there.)
+        if (nb) {
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            const uint32_t c = tid + r * NT;
-            if (c < nch) load4<T>(cip, vals, a0 + 4 * c, cv[r], vv[r]);
+            for (int r = 0; r < R; r++) load4<T>(cip, vals, a0 + 4 * min((uint32_t)(tid + r * NT), nch - 1), cv[r], vv[r]);
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t t4 = 4 * (tid + r * NT) + i;  // lead + tile entry
+                    xv[r][i] = gather(t4 >= lead && t4 - lead < nb ? cv[r][i] : 0u);
+                }
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t t4 = 4 * (tid + r * NT) + i;
+                    if (t4 >= lead && t4 - lead < nb) store(t4 - lead, cv[r][i], vv[r][i], xv[r][i]);
+                }
         }
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t t4 = 4 * (tid + r * NT) + i;  // lead + tile entry
-                if (t4 >= lead && t4 - lead < nb) xv[r][i] = gather(cv[r][i]);
-            }
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t t4 = 4 * (tid + r * NT) + i;
-                if (t4 >= lead && t4 - lead < nb) store(t4 - lead, cv[r][i], vv[r][i], xv[r][i]);
-            }
         if (tid == NT - 1 && (uint32_t)(R * NT) < nch) {  // one more chunk: lead + nb > 4 R NT
             uint32_t cw[4];
             T vw[4];
@@ -902,10 +906,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH ? 6 : 1))
         const uint32_t a0 = e0 & ~3u, lead = e0 - a0, nch = (lead + n + 3) >> 2;
         uint32_t rv[R][4], cv[R][4];
         T vv[R][4], xv[R][4];
+        if (n) {  // no load under a branch, as in csr_merge_kernel's chunked path
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            const uint32_t c = tid + r * NT;
-            if (c < nch) {
+            for (int r = 0; r < R; r++) {
+                const uint32_t c = min((uint32_t)(tid + r * NT), nch - 1);
                 const u32x4 w = __builtin_nontemporal_load((gptr<const u32x4>)(rip + a0 + 4 * c));
                 rv[r][0] = w[0];
                 rv[r][1] = w[1];
@@ -913,21 +917,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH ? 6 : 1))
                 rv[r][3] = w[3];
                 load4<T>(cip, vals, a0 + 4 * c, cv[r], vv[r]);
             }
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t t4 = 4 * (tid + r * NT) + i;  // lead + tile entry
+                    xv[r][i] = gather(t4 >= lead && t4 - lead < n ? cv[r][i] : 0u);
+                }
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t t4 = 4 * (tid + r * NT) + i;
+                    if (t4 >= lead && t4 - lead < n) store(t4 - lead, rv[r][i], cv[r][i], vv[r][i], xv[r][i]);
+                }
         }
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t t4 = 4 * (tid + r * NT) + i;  // lead + tile entry
-                if (t4 >= lead && t4 - lead < n) xv[r][i] = gather(cv[r][i]);
-            }
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t t4 = 4 * (tid + r * NT) + i;
-                if (t4 >= lead && t4 - lead < n) store(t4 - lead, rv[r][i], cv[r][i], vv[r][i], xv[r][i]);
-            }
         if (tid == NT - 1 && (uint32_t)(R * NT) < nch) {  // one more chunk: lead + n > 4 R NT
             const uint32_t a = a0 + 4 * (uint32_t)(R * NT);
             const u32x4 w = __builtin_nontemporal_load((gptr<const u32x4>)(rip + a));
