@@ -138,6 +138,25 @@ __device__ __host__ inline uint32_t merge_search(uint32_t d, uint32_t na, uint32
     return lo;
 }
 
+// N contiguous values of x at p, which the caller knows to be aligned to the
+// load width (16 or 8 bytes): wide loads without a runtime alignment check,
+// so that no load sits under a branch.
+template <typename T, int N> __device__ __forceinline__ void ld_al(gptr<const T> p, T (&out)[N]) {
+    constexpr int B = N * (int)sizeof(T);
+    if constexpr (B % 16 == 0) {
+        u32x4 w[B / 16];
+#pragma unroll
+        for (int i = 0; i < B / 16; i++) w[i] = ((gptr<const u32x4>)p)[i];
+        __builtin_memcpy(out, w, B);
+    } else if constexpr (B == 8) {
+        const u32x2 w = *(gptr<const u32x2>)p;
+        __builtin_memcpy(out, &w, B);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) out[i] = p[i];
+    }
+}
+
 // Four consecutive entries a .. a+3 of a (column index, value) stream (a a
 // multiple of 4, both arrays aligned to 4 entries and readable 3 entries
 // past their end): one 16-byte index load and one 4 * sizeof(T) value load.
@@ -1667,11 +1686,18 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
         n = cnnz[ci + 1] - b0;
         next_brow = NONE;  // unused: completeness comes from the row ends
         const uint32_t nrl = min(na + 1, P.nrows - i0);  // the ends of rows i0 .. i0 + na
+        // all loads first (lanes past an end load its last element again), then the
+        // LDS stores: a load under a branch would wait for the loads before it
+        uint32_t re[IPT + 1], bc[IPT];
+#pragma unroll
+        for (int q = 0; q <= IPT; q++) re[q] = nrl ? rp[i0 + min((uint32_t)(tid + q * TPB), nrl - 1) + 1] : 0u;
+#pragma unroll
+        for (int q = 0; q < IPT; q++) bc[q] = n ? ldg(cip + b0 + min((uint32_t)(tid + q * TPB), n - 1)) : 0u;
 #pragma unroll
         for (int q = 0; q <= IPT; q++) {
             const uint32_t k = tid + q * TPB;
-            if (k < nrl) s_rowend[k] = rp[i0 + k + 1];
-            if (q < IPT && k < n) s_bcol[k] = ldg(cip + b0 + k);
+            if (k < nrl) s_rowend[k] = re[q];
+            if (q < IPT && k < n) s_bcol[k] = bc[q];
         }
         __syncthreads();
         // row of block b0 + k: i0 + the number of the tile's row ends at or before it
@@ -1692,17 +1718,34 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
         b0 = cent[ci];
         n = cent[ci + 1] - b0;
         next_brow = (b0 + n < P.nent) ? rip[b0 + n] : NONE;
+        if (n) {  // loads first, then the LDS stores, as above
+            uint32_t br[IPT], bc[IPT];
 #pragma unroll
-        for (int q = 0; q < IPT; q++) {
-            const uint32_t k = tid + q * TPB;
-            if (k < n) {
-                s_brow[k] = ldg(rip + b0 + k);
-                s_bcol[k] = ldg(cip + b0 + k);
+            for (int q = 0; q < IPT; q++) {
+                const uint32_t k = min((uint32_t)(tid + q * TPB), n - 1);
+                br[q] = ldg(rip + b0 + k);
+                bc[q] = ldg(cip + b0 + k);
+            }
+#pragma unroll
+            for (int q = 0; q < IPT; q++) {
+                const uint32_t k = tid + q * TPB;
+                if (k < n) {
+                    s_brow[k] = br[q];
+                    s_bcol[k] = bc[q];
+                }
             }
         }
     }
     __syncthreads();
     using CO = Coal<T, R, C>;
+    // x read in whole pieces without bound checks when every block column is whole and x is
+    // contiguous and aligned (uniform over the tile; a template argument, so that no load
+    // sits under a branch)
+    constexpr int XW = (CO::ok && CO::G > 1) ? CO::CH : C;  // x values per piece
+    const bool fast = ldx == 1 && P.ncols_elem % C == 0 &&
+                      ((uintptr_t)io.x(pid, P) % (XW * sizeof(T) >= 16 ? 16 : XW * sizeof(T))) == 0;
+    auto phase1 = [&](auto fast_c) {
+    constexpr bool FAST = decltype(fast_c)::value;
     if constexpr (CO::ok && CO::G > 1) {
         // phase 1, blocks wider than 16 bytes: lanes take 16-byte chunks of
         // consecutive blocks (one load instruction covers 1 KB of values), and
@@ -1711,6 +1754,9 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
             const uint32_t nl = n - 1;
             const int g = tid % CO::G;  // TPB is a multiple of G: a lane keeps its chunk slot
             const int c0 = (g % CO::CPR) * CO::CH;
+            // every lane stores its sum, past the tile's end too (those slots of s_part are
+            // never read): a store under a branch between two iterations would hold the
+            // next iteration's loads back
 #pragma unroll
             for (int q = 0; q < IPT * CO::G; q++) {
                 const uint32_t blk = (uint32_t)(tid + q * TPB) / CO::G;
@@ -1718,7 +1764,9 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
                 const uint32_t col = s_bcol[bc] * C + c0;
                 T v[CO::CH], xs[CO::CH];
                 ldg_n<T, CO::CH>(bv + (size_t)(b0 + bc) * (R * C) + g * CO::CH, v);
-                if (ldx == 1 && col + CO::CH <= P.ncols_elem) {
+                if constexpr (FAST) {
+                    ld_al<T, CO::CH>(x + col, xs);
+                } else if (ldx == 1 && col + CO::CH <= P.ncols_elem) {
                     ld_n<T, CO::CH>(x + col, xs);
                 } else {
 #pragma unroll
@@ -1729,39 +1777,45 @@ __global__ __launch_bounds__(TPB) void bcoo_seg_kernel(const Prob *__restrict__ 
 #pragma unroll
                 for (int j = 0; j < CO::CH; j++) p = p + mul<T>(v[j], xs[j]);
                 p = xor_sum<1, CO::CPR>(p);
-                if (blk < n && g % CO::CPR == 0) s_part[blk * R + g / CO::CPR] = p;
+                if (g % CO::CPR == 0) s_part[blk * R + g / CO::CPR] = p;
             }
         }
     } else {
     // phase 1: a thread takes whole blocks (R*C contiguous values in wide loads,
     // the C x values in one load when aligned) and forms their R row sums
+    if (n) {
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
         const uint32_t blk = tid + q * TPB;
-        if (blk < n) {
-            const uint32_t J = s_bcol[blk];
-            T v[R * C];
-            ldg_n<T, R * C>(bv + (size_t)(b0 + blk) * (R * C), v);
-            T xs[C];
-            if (ldx == 1 && J * C + C <= P.ncols_elem) {
-                ld_n<T, C>(x + (size_t)J * C, xs);
-            } else {
+        const uint32_t bc = min(blk, n - 1);  // lanes past the tile reload its last block
+        const uint32_t J = s_bcol[bc];
+        T v[R * C];
+        ldg_n<T, R * C>(bv + (size_t)(b0 + bc) * (R * C), v);
+        T xs[C];
+        if constexpr (FAST) {
+            ld_al<T, C>(x + (size_t)J * C, xs);
+        } else if (ldx == 1 && J * C + C <= P.ncols_elem) {
+            ld_n<T, C>(x + (size_t)J * C, xs);
+        } else {
 #pragma unroll
-                for (int c = 0; c < C; c++) {
-                    const uint32_t col = J * C + c;
-                    xs[c] = col < P.ncols_elem ? x[(size_t)col * ldx] : T(0);
-                }
+            for (int c = 0; c < C; c++) {
+                const uint32_t col = J * C + c;
+                xs[c] = col < P.ncols_elem ? x[(size_t)col * ldx] : T(0);
             }
+        }
 #pragma unroll
-            for (int i = 0; i < R; i++) {
-                A p = A(0);
+        for (int i = 0; i < R; i++) {  // past the tile's end too, as above
+            A p = A(0);
 #pragma unroll
-                for (int c = 0; c < C; c++) p = p + mul<T>(v[i * C + c], xs[c]);
-                s_part[blk * R + i] = p;
-            }
+            for (int c = 0; c < C; c++) p = p + mul<T>(v[i * C + c], xs[c]);
+            s_part[blk * R + i] = p;
         }
     }
     }
+    }
+    };
+    if (fast) phase1(std::true_type{});
+    else phase1(std::false_type{});
     __syncthreads();
 
     const uint32_t a = min((uint32_t)tid * IPT, n), bb = min(a + IPT, n);
