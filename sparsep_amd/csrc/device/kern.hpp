@@ -1177,6 +1177,22 @@ __global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
     const uint32_t ncols = P.ncols_elem;
+    // x read in whole pieces without bound checks when every block column is whole and x is
+    // contiguous and aligned (uniform over the unit; a template argument, so that no load
+    // sits under a branch)
+    const bool fast = ldx == 1 && ncols % C == 0 &&
+                      ((uintptr_t)io.x(pid, P) % (CO::CH * sizeof(T) >= 16 ? 16 : CO::CH * sizeof(T))) == 0;
+    auto xpiece = [&](auto fast_c, uint32_t col, T (&xs)[CO::CH]) {
+        if constexpr (decltype(fast_c)::value) {
+            ld_al<T, CO::CH>(x + col, xs);
+        } else if (ldx == 1 && col + CO::CH <= ncols) {
+            ld_n<T, CO::CH>(x + col, xs);
+        } else {
+#pragma unroll
+            for (int j = 0; j < CO::CH; j++) xs[j] = col + j < ncols ? x[(size_t)(col + j) * ldx] : T(0);
+        }
+    };
+    auto rows = [&](auto fast_c) {
     for (uint32_t I = r0 + wid; I < r1; I += NW) {
         const uint32_t b = rp[I], e = rp[I + 1];
         A acc = A(0);
@@ -1192,16 +1208,7 @@ __global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__
 #pragma unroll
             for (int u = 0; u < 4; u++) ldg_n<T, CO::CH>(bv + (size_t)(k + u * CO::PB) * (R * C) + g * CO::CH, v[u]);
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t col = J[u] * C + c0;
-                if (ldx == 1 && col + CO::CH <= ncols) {
-                    ld_n<T, CO::CH>(x + col, xs[u]);
-                } else {
-#pragma unroll
-                    for (int j = 0; j < CO::CH; j++)
-                        xs[u][j] = col + j < ncols ? x[(size_t)(col + j) * ldx] : T(0);
-                }
-            }
+            for (int u = 0; u < 4; u++) xpiece(fast_c, J[u] * C + c0, xs[u]);
 #pragma unroll
             for (int u = 0; u < 4; u++)
 #pragma unroll
@@ -1218,22 +1225,13 @@ __global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__
 #pragma unroll
             for (int u = 0; u < U; u++) ldg_n<T, CO::CH>(bv + (size_t)min(k + u * CO::PB, e - 1) * (R * C) + g * CO::CH, v[u]);
 #pragma unroll
+            for (int u = 0; u < U; u++) xpiece(fast_c, J[u] * C + c0, xs[u]);
+#pragma unroll
             for (int u = 0; u < U; u++) {
-                const uint32_t col = J[u] * C + c0;
-                if (ldx == 1 && col + CO::CH <= ncols) {
-                    ld_n<T, CO::CH>(x + col, xs[u]);
-                } else {
+                const bool in = k + u * CO::PB < e;  // a select, not a branch
 #pragma unroll
-                    for (int j = 0; j < CO::CH; j++)
-                        xs[u][j] = col + j < ncols ? x[(size_t)(col + j) * ldx] : T(0);
-                }
+                for (int j = 0; j < CO::CH; j++) acc = fma_acc<T>(in ? v[u][j] : T(0), xs[u][j], acc);
             }
-#pragma unroll
-            for (int u = 0; u < U; u++)
-                if (k + u * CO::PB < e) {
-#pragma unroll
-                    for (int j = 0; j < CO::CH; j++) acc = fma_acc<T>(v[u][j], xs[u][j], acc);
-                }
         }
         // lanes of row i: all chunks of the row (low bits) and all block slots (high bits)
         acc = xor_sum<1, CO::CPR>(acc);
@@ -1243,6 +1241,9 @@ __global__ __launch_bounds__(TPB) void bcsr_coal_kernel(const Prob *__restrict__
             if (row < P.nrows_elem) y[(size_t)row * ldy] = to_val<T>(acc);
         }
     }
+    };
+    if (fast) rows(std::true_type{});
+    else rows(std::false_type{});
     }
 }
 
