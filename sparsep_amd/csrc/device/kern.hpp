@@ -1356,6 +1356,7 @@ __global__ __launch_bounds__(TPB) void rows_rhs_kernel(const Prob *__restrict__ 
 // ---------------------------------------------------------------------------
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 template <typename T, int K>
 __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__ probs,
@@ -1522,11 +1523,18 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
 // block g of the step (8 blocks per step) and right-hand sides 4r .. 4r + 3:
 // it reads the block's 16 values from LDS (the 8 lanes of a block read the
 // same words) and the four x rows' 16-byte pieces it needs, and keeps a
-// 4 x 4 tile of sums, added over the 8 block slots at the end of the row.
-// Measured on the 27-point stencil (64^3 x 4): 409-417 us against 511-515 us
-// for the matrix-core kernel, whose 4-byte x loads (one column per lane, as
-// the 4 x 4 x 1 instruction wants its B operand) issue four times as many
-// load instructions; at 8 columns the matrix-core kernel stays faster
+// 4 x 4 tile of sums.  At the end of the row the 8 block slots' tiles are
+// added by a scattered butterfly (each exchange hands over half of the sums a
+// lane keeps: 14 lane exchanges instead of 48), after which every lane stores
+// 8 bytes of y.  When every block column is whole the x rows load unchecked
+// and an empty slot of the last step skips its products; otherwise empty
+// slots read a block of zeros and x rows past the matrix read as zeros.
+// The kernel issues its vector ALU instructions most of the time (counters on
+// the 27-point stencil 64^3 x 4: 179M instructions, 86% of the SIMD cycles,
+// before the two changes above): they took it from 409-417 us to 345 us.
+// The matrix-core kernel takes 511-515 us: its 4-byte x loads (one column per
+// lane, as the 4 x 4 x 1 instruction wants its B operand) issue four times as
+// many load instructions; at 8 columns the matrix-core kernel stays faster
 // (207-213 against 250-264 us).
 // ---------------------------------------------------------------------------
 template <int K>
@@ -1538,10 +1546,12 @@ __global__ __launch_bounds__(TPB) void bcsr4_valu_kernel(const Prob *__restrict_
     constexpr int NW = TPB / WAVE;
     constexpr int GS = K / 4;     // lanes per block
     constexpr int G = WAVE / GS;  // blocks per step
-    __shared__ __align__(16) float s_val[NW][WAVE * 16];  // a wave's batch of 64 blocks
+    // a wave's batch of 64 blocks, then a block of zeros that the empty slots of a step read
+    __shared__ __align__(16) float s_val[NW][(WAVE + 1) * 16];
     const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
     const int g = lane / GS, r4 = lane % GS;
     float *const sv = s_val[wid];
+    if (lane < 16) sv[WAVE * 16 + lane] = 0.f;
     for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const uint32_t pid = unit_pid[u];
     const Prob P = probs[pid];
@@ -1556,8 +1566,9 @@ __global__ __launch_bounds__(TPB) void bcsr4_valu_kernel(const Prob *__restrict_
     // template argument so that no load sits under a branch
     const bool xw = ((uintptr_t)io.x(pid, P) % 16 == 0) && (ldx % 4 == 0);
     const bool yw = ((uintptr_t)io.y(pid, P) % 16 == 0) && (ldy % 4 == 0);
-    auto rows = [&](auto xw_c) {
+    auto rows = [&](auto xw_c, auto wh_c) {
         constexpr bool XW = decltype(xw_c)::value;
+        constexpr bool WH = decltype(wh_c)::value;  // whole 16-byte x pieces and whole block columns
         for (uint32_t I = r0 + wid; I < r1; I += NW) {
             const uint32_t b = rp[I], e = rp[I + 1];
             float acc[4][4];
@@ -1578,17 +1589,42 @@ __global__ __launch_bounds__(TPB) void bcsr4_valu_kernel(const Prob *__restrict_
                 const uint32_t Jl = (uint32_t)lane < nb ? ldg(cip + base + lane) : 0u;
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
+                if constexpr (WH) {
+                    // whole block columns: the block's four x rows load unchecked and an empty
+                    // slot skips its products (two blocks per lane and step, all eight x rows in
+                    // flight, measured slower: 122 VGPRs leave four waves per SIMD instead of five)
+                    for (uint32_t t0 = 0; t0 < nb; t0 += G) {
+                        const uint32_t sl = t0 + g;
+                        const bool act = sl < nb;
+                        const uint32_t J = (uint32_t)__shfl((int)Jl, (int)(act ? sl : 0u), WAVE);
+                        const gptr<const float> px = x + (size_t)J * 4 * ldx + 4 * r4;
+                        f32x4 xv[4], av[4];
+#pragma unroll
+                        for (int c = 0; c < 4; c++) xv[c] = *(gptr<const f32x4>)(px + (size_t)c * ldx);
+#pragma unroll
+                        for (int i = 0; i < 4; i++) av[i] = *(const f32x4 *)(sv + (act ? sl : 0u) * 16 + i * 4);
+                        if (act) {
+#pragma unroll
+                            for (int i = 0; i < 4; i++)
+#pragma unroll
+                                for (int c = 0; c < 4; c++)
+#pragma unroll
+                                    for (int q = 0; q < 4; q++) acc[i][q] = __builtin_fmaf(av[i][c], xv[c][q], acc[i][q]);
+                        }
+                    }
+                } else {
                 for (uint32_t t0 = 0; t0 < nb; t0 += G) {
                     const uint32_t sl = t0 + g;
                     const bool act = sl < nb;
-                    const uint32_t sc = act ? sl : 0u;  // a valid block; its products are zeroed
-                    const uint32_t J = (uint32_t)__shfl((int)Jl, (int)sc, WAVE);
+                    const uint32_t J = (uint32_t)__shfl((int)Jl, (int)(act ? sl : 0u), WAVE);
+                    // an empty slot multiplies the zero block by zeros
+                    const float *const pa = sv + (act ? sl : (uint32_t)WAVE) * 16;
                     float a[16], xr[4][4];
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
-                        const f32x4 v = *(const f32x4 *)(sv + sc * 16 + i * 4);
+                        const f32x4 v = *(const f32x4 *)(pa + i * 4);
 #pragma unroll
-                        for (int c = 0; c < 4; c++) a[i * 4 + c] = act ? v[c] : 0.f;
+                        for (int c = 0; c < 4; c++) a[i * 4 + c] = v[c];
                     }
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
@@ -1611,31 +1647,46 @@ __global__ __launch_bounds__(TPB) void bcsr4_valu_kernel(const Prob *__restrict_
 #pragma unroll
                             for (int q = 0; q < 4; q++) acc[i][q] = __builtin_fmaf(a[i * 4 + c], xr[c][q], acc[i][q]);
                 }
+                }
                 __builtin_amdgcn_wave_barrier();  // the next batch overwrites the staged values
             }
+            // sums over the 8 block slots, scattered: each exchange halves the values a lane
+            // keeps (block rows by slot bits 2 and 1, then column pairs by slot bit 0), so lane
+            // (g, r) ends with the sums of block row (g >> 1) & 3 at columns 4r + 2(g & 1) + {0, 1}
+            const bool g2 = g & 4, g1 = g & 2, g0 = g & 1;
+            float h[2][4], c[4], d[2];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc[i][q] = xor_sum<GS, WAVE>(acc[i][q]);
-            if (g == 0) {
+                for (int q = 0; q < 4; q++) {
+                    const float keep = g2 ? acc[i + 2][q] : acc[i][q], send = g2 ? acc[i][q] : acc[i + 2][q];
+                    h[i][q] = keep + lane_xor<32>(send);
+                }
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const uint32_t row = I * 4 + i;
-                    if (row >= P.nrows_elem) continue;
-                    const gptr<float> py = y + (size_t)row * ldy + 4 * r4;
-                    if (yw) {
-                        const f32x4 v = {acc[i][0], acc[i][1], acc[i][2], acc[i][3]};
-                        *(gptr<f32x4>)py = v;
-                    } else {
+            for (int q = 0; q < 4; q++) {
+                const float keep = g1 ? h[1][q] : h[0][q], send = g1 ? h[0][q] : h[1][q];
+                c[q] = keep + lane_xor<16>(send);
+            }
 #pragma unroll
-                        for (int q = 0; q < 4; q++) py[q] = acc[i][q];
-                    }
+            for (int q = 0; q < 2; q++) {
+                const float keep = g0 ? c[q + 2] : c[q], send = g0 ? c[q] : c[q + 2];
+                d[q] = keep + lane_xor<8>(send);
+            }
+            const uint32_t row = I * 4 + (g1 ? 1 : 0) + (g2 ? 2 : 0);
+            if (row < P.nrows_elem) {
+                const gptr<float> py = y + (size_t)row * ldy + 4 * r4 + (g0 ? 2 : 0);
+                if (yw) {
+                    *(gptr<f32x2>)py = f32x2{d[0], d[1]};
+                } else {
+                    py[0] = d[0];
+                    py[1] = d[1];
                 }
             }
         }
     };
-    if (xw) rows(std::true_type{});
-    else rows(std::false_type{});
+    if (xw && ncols % 4 == 0) rows(std::true_type{}, std::true_type{});
+    else if (xw) rows(std::true_type{}, std::false_type{});
+    else rows(std::false_type{}, std::false_type{});
     }
 }
 
