@@ -1515,168 +1515,186 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// BCSR 4 x 4 FP32 blocks times 32 right-hand sides on the vector ALUs (the
-// block-row product of spmv/1D/BCSR-nnz/dpu/task.c:337-370 over a 4 x 32
-// panel of x per block).  A wave takes a block row; it stages the values of
-// its blocks in LDS 64 blocks at a time (coalesced 16-byte loads) and hands
-// their block columns to the lanes by shuffles.  Lane l = 8 g + r takes
-// block g of the step (8 blocks per step) and right-hand sides 4r .. 4r + 3:
-// it reads the block's 16 values from LDS (the 8 lanes of a block read the
-// same words) and the four x rows' 16-byte pieces it needs, and keeps a
-// 4 x 4 tile of sums.  At the end of the row the 8 block slots' tiles are
+// BCSR 4 x 4 FP32 or FP64 blocks times 32 right-hand sides on the vector
+// ALUs (the block-row product of spmv/1D/BCSR-nnz/dpu/task.c:337-370 over a
+// 4 x 32 panel of x per block).  A wave takes a block row; it stages the
+// values of its blocks in LDS 4 KB at a time (coalesced 16-byte loads) and
+// hands their block columns to the lanes by shuffles.  With V values per
+// 16-byte piece (4 FP32, 2 FP64), lane l = (32 / V) g + r takes block g of the
+// step (8 FP32 or 4 FP64 blocks per step) and right-hand sides V r .. V r +
+// V - 1: it reads the block's 16 values from LDS (the lanes of a block read
+// the same words) and the four x rows' 16-byte pieces it needs, and keeps a
+// 4 x V tile of sums.  At the end of the row the block slots' tiles are
 // added by a scattered butterfly (each exchange hands over half of the sums a
-// lane keeps: 14 lane exchanges instead of 48), after which every lane stores
-// 8 bytes of y.  When every block column is whole the x rows load unchecked
+// lane keeps: FP32 14 lane exchanges instead of 48), after which every lane
+// stores two values of y.  When every block column is whole the x rows load unchecked
 // and an empty slot of the last step skips its products; otherwise empty
 // slots read a block of zeros and x rows past the matrix read as zeros.
-// The kernel issues its vector ALU instructions most of the time (counters on
-// the 27-point stencil 64^3 x 4: 179M instructions, 86% of the SIMD cycles,
-// before the two changes above): they took it from 409-417 us to 345 us.
-// The matrix-core kernel takes 511-515 us: its 4-byte x loads (one column per
+// The FP32 kernel issues its vector ALU instructions most of the time
+// (counters on the 27-point stencil 64^3 x 4: 179M instructions, 86% of the
+// SIMD cycles, before the two changes above): they took it from 409-417 us to
+// 345 us.  The FP32 matrix-core kernel takes 511-515 us: its 4-byte x loads (one column per
 // lane, as the 4 x 4 x 1 instruction wants its B operand) issue four times as
 // many load instructions; at 8 columns the matrix-core kernel stays faster
 // (207-213 against 250-264 us).
 // ---------------------------------------------------------------------------
-template <int K>
+template <typename T, int K>
 __global__ __launch_bounds__(TPB) void bcsr4_valu_kernel(const Prob *__restrict__ probs,
                                                          const uint32_t *__restrict__ unit_pid,
-                                                         const uint32_t *__restrict__ urow, Ptrs<float> io,
+                                                         const uint32_t *__restrict__ urow, Ptrs<T> io,
                                                          uint32_t nunits) {
     static_assert(K == 32, "32 right-hand sides per pass");
+    static_assert(__is_same(T, float) || __is_same(T, double), "FP32 or FP64 blocks");
     constexpr int NW = TPB / WAVE;
-    constexpr int GS = K / 4;     // lanes per block
-    constexpr int G = WAVE / GS;  // blocks per step
-    // a wave's batch of 64 blocks, then a block of zeros that the empty slots of a step read
-    __shared__ __align__(16) float s_val[NW][(WAVE + 1) * 16];
+    constexpr int V = 16 / (int)sizeof(T);  // values per 16-byte piece
+    constexpr int GS = K / V;               // lanes per block
+    constexpr int G = WAVE / GS;            // blocks per step (8 FP32, 4 FP64)
+    constexpr int NB = 4096 / (16 * (int)sizeof(T));  // blocks per staged batch (4 KB)
+    constexpr int PPB = (int)sizeof(T);               // 16-byte pieces per block
+    static_assert(NB * PPB == 4 * WAVE, "four pieces per lane and batch");
+    typedef T VT __attribute__((ext_vector_type(V)));
+    // a wave's batch of blocks, then a block of zeros that the empty slots of a step read
+    __shared__ __align__(16) T s_val[NW][(NB + 1) * 16];
     const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
-    const int g = lane / GS, r4 = lane % GS;
-    float *const sv = s_val[wid];
-    if (lane < 16) sv[WAVE * 16 + lane] = 0.f;
+    const int g = lane / GS, r = lane % GS;
+    T *const sv = s_val[wid];
+    if (lane < 16) sv[NB * 16 + lane] = T(0);
     for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const uint32_t pid = unit_pid[u];
     const Prob P = probs[pid];
-    const gptr<const float> __restrict__ x = gp(io.x(pid, P));
-    const gptr<float> __restrict__ y = gp(io.y(pid, P));
+    const gptr<const T> __restrict__ x = gp(io.x(pid, P));
+    const gptr<T> __restrict__ y = gp(io.y(pid, P));
     const gptr<const uint32_t> __restrict__ rp = gp(P.ptr), cip = gp(P.cind);
-    const gptr<const float> __restrict__ bv = gp((const float *)P.val);
+    const gptr<const T> __restrict__ bv = gp((const T *)P.val);
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
     const uint32_t ncols = P.ncols_elem;
     // whole 16-byte pieces of x and y rows (uniform over the launch); the x case is a
     // template argument so that no load sits under a branch
-    const bool xw = ((uintptr_t)io.x(pid, P) % 16 == 0) && (ldx % 4 == 0);
-    const bool yw = ((uintptr_t)io.y(pid, P) % 16 == 0) && (ldy % 4 == 0);
+    const bool xw = ((uintptr_t)io.x(pid, P) % 16 == 0) && (ldx % V == 0);
+    const bool yw = ((uintptr_t)io.y(pid, P) % 16 == 0) && (ldy % V == 0);
     auto rows = [&](auto xw_c, auto wh_c) {
         constexpr bool XW = decltype(xw_c)::value;
         constexpr bool WH = decltype(wh_c)::value;  // whole 16-byte x pieces and whole block columns
         for (uint32_t I = r0 + wid; I < r1; I += NW) {
             const uint32_t b = rp[I], e = rp[I + 1];
-            float acc[4][4];
+            T acc[4][V];
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc[i][q] = 0.f;
-            for (uint32_t base = b; base < e; base += WAVE) {
-                const uint32_t nb = min((uint32_t)WAVE, e - base);
+                for (int q = 0; q < V; q++) acc[i][q] = T(0);
+            for (uint32_t base = b; base < e; base += NB) {
+                const uint32_t nb = min((uint32_t)NB, e - base);
                 // the batch's values to LDS in 16-byte pieces, its block columns to lanes
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int c0 = 0; c0 < 4; c0++) {
                     const uint32_t c = (uint32_t)lane + (uint32_t)c0 * WAVE;  // piece c of the batch
-                    if (c < nb * 4)
-                        *(u32x4 *)(sv + c * 4) = __builtin_nontemporal_load((gptr<const u32x4>)(bv + (size_t)base * 16) + c);
+                    if (c < nb * PPB)
+                        *(u32x4 *)(sv + c * V) = __builtin_nontemporal_load((gptr<const u32x4>)(bv + (size_t)base * 16) + c);
                 }
                 const uint32_t Jl = (uint32_t)lane < nb ? ldg(cip + base + lane) : 0u;
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 if constexpr (WH) {
                     // whole block columns: the block's four x rows load unchecked and an empty
-                    // slot skips its products (two blocks per lane and step, all eight x rows in
-                    // flight, measured slower: 122 VGPRs leave four waves per SIMD instead of five)
+                    // slot skips its products (FP32: two blocks per lane and step, all eight x
+                    // rows in flight, measured slower: 122 VGPRs leave four waves per SIMD
+                    // instead of five)
                     for (uint32_t t0 = 0; t0 < nb; t0 += G) {
                         const uint32_t sl = t0 + g;
                         const bool act = sl < nb;
                         const uint32_t J = (uint32_t)__shfl((int)Jl, (int)(act ? sl : 0u), WAVE);
-                        const gptr<const float> px = x + (size_t)J * 4 * ldx + 4 * r4;
-                        f32x4 xv[4], av[4];
+                        const gptr<const T> px = x + (size_t)J * 4 * ldx + V * r;
+                        VT xv[4], av[16 / V];
 #pragma unroll
-                        for (int c = 0; c < 4; c++) xv[c] = *(gptr<const f32x4>)(px + (size_t)c * ldx);
+                        for (int c = 0; c < 4; c++) xv[c] = *(gptr<const VT>)(px + (size_t)c * ldx);
 #pragma unroll
-                        for (int i = 0; i < 4; i++) av[i] = *(const f32x4 *)(sv + (act ? sl : 0u) * 16 + i * 4);
+                        for (int p = 0; p < 16 / V; p++) av[p] = *(const VT *)(sv + (act ? sl : 0u) * 16 + p * V);
                         if (act) {
 #pragma unroll
                             for (int i = 0; i < 4; i++)
 #pragma unroll
                                 for (int c = 0; c < 4; c++)
 #pragma unroll
-                                    for (int q = 0; q < 4; q++) acc[i][q] = __builtin_fmaf(av[i][c], xv[c][q], acc[i][q]);
+                                    for (int q = 0; q < V; q++)
+                                        acc[i][q] = fma_acc<T>(av[(i * 4 + c) / V][(i * 4 + c) % V], xv[c][q], acc[i][q]);
                         }
                     }
                 } else {
-                for (uint32_t t0 = 0; t0 < nb; t0 += G) {
-                    const uint32_t sl = t0 + g;
-                    const bool act = sl < nb;
-                    const uint32_t J = (uint32_t)__shfl((int)Jl, (int)(act ? sl : 0u), WAVE);
-                    // an empty slot multiplies the zero block by zeros
-                    const float *const pa = sv + (act ? sl : (uint32_t)WAVE) * 16;
-                    float a[16], xr[4][4];
+                    for (uint32_t t0 = 0; t0 < nb; t0 += G) {
+                        const uint32_t sl = t0 + g;
+                        const bool act = sl < nb;
+                        const uint32_t J = (uint32_t)__shfl((int)Jl, (int)(act ? sl : 0u), WAVE);
+                        // an empty slot multiplies the zero block by zeros
+                        const T *const pa = sv + (act ? sl : (uint32_t)NB) * 16;
+                        T a[16], xr[4][V];
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const f32x4 v = *(const f32x4 *)(pa + i * 4);
+                        for (int p = 0; p < 16 / V; p++) {
+                            const VT v = *(const VT *)(pa + p * V);
 #pragma unroll
-                        for (int c = 0; c < 4; c++) a[i * 4 + c] = v[c];
-                    }
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const uint32_t col = J * 4 + c;
-                        const bool ok = act && col < ncols;
-                        const gptr<const float> px = x + (size_t)(ok ? col : 0u) * ldx + 4 * r4;
-                        if constexpr (XW) {
-                            const f32x4 v = *(gptr<const f32x4>)px;
-#pragma unroll
-                            for (int q = 0; q < 4; q++) xr[c][q] = ok ? v[q] : 0.f;
-                        } else {
-#pragma unroll
-                            for (int q = 0; q < 4; q++) xr[c][q] = ok ? px[q] : 0.f;
+                            for (int q = 0; q < V; q++) a[p * V + q] = v[q];
                         }
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            const uint32_t col = J * 4 + c;
+                            const bool ok = act && col < ncols;
+                            const gptr<const T> px = x + (size_t)(ok ? col : 0u) * ldx + V * r;
+                            if constexpr (XW) {
+                                const VT v = *(gptr<const VT>)px;
+#pragma unroll
+                                for (int q = 0; q < V; q++) xr[c][q] = ok ? v[q] : T(0);
+                            } else {
+#pragma unroll
+                                for (int q = 0; q < V; q++) xr[c][q] = ok ? px[q] : T(0);
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+#pragma unroll
+                            for (int c = 0; c < 4; c++)
+#pragma unroll
+                                for (int q = 0; q < V; q++) acc[i][q] = fma_acc<T>(a[i * 4 + c], xr[c][q], acc[i][q]);
                     }
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-#pragma unroll
-                        for (int c = 0; c < 4; c++)
-#pragma unroll
-                            for (int q = 0; q < 4; q++) acc[i][q] = __builtin_fmaf(a[i * 4 + c], xr[c][q], acc[i][q]);
-                }
                 }
                 __builtin_amdgcn_wave_barrier();  // the next batch overwrites the staged values
             }
-            // sums over the 8 block slots, scattered: each exchange halves the values a lane
-            // keeps (block rows by slot bits 2 and 1, then column pairs by slot bit 0), so lane
-            // (g, r) ends with the sums of block row (g >> 1) & 3 at columns 4r + 2(g & 1) + {0, 1}
-            const bool g2 = g & 4, g1 = g & 2, g0 = g & 1;
-            float h[2][4], c[4], d[2];
+            // sums over the G block slots, scattered: each exchange hands over half of the sums
+            // a lane keeps (block rows by lane bits 5 and 4; FP32 then column pairs by lane bit
+            // 3), so every lane ends with two sums of block row (lane >> 4) & 3, at columns
+            // V r and V r + 1 (FP32: 4 r + 2 and 4 r + 3 when lane bit 3 is set)
+            const bool hb = lane & 32, mb = lane & 16, lb = lane & 8;
+            T h[2][V], c[V], d[2];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float keep = g2 ? acc[i + 2][q] : acc[i][q], send = g2 ? acc[i][q] : acc[i + 2][q];
+                for (int q = 0; q < V; q++) {
+                    const T keep = hb ? acc[i + 2][q] : acc[i][q], send = hb ? acc[i][q] : acc[i + 2][q];
                     h[i][q] = keep + lane_xor<32>(send);
                 }
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const float keep = g1 ? h[1][q] : h[0][q], send = g1 ? h[0][q] : h[1][q];
+            for (int q = 0; q < V; q++) {
+                const T keep = mb ? h[1][q] : h[0][q], send = mb ? h[0][q] : h[1][q];
                 c[q] = keep + lane_xor<16>(send);
             }
+            uint32_t col0 = V * r;
+            if constexpr (V == 4) {
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const float keep = g0 ? c[q + 2] : c[q], send = g0 ? c[q] : c[q + 2];
-                d[q] = keep + lane_xor<8>(send);
+                for (int q = 0; q < 2; q++) {
+                    const T keep = lb ? c[q + 2] : c[q], send = lb ? c[q] : c[q + 2];
+                    d[q] = keep + lane_xor<8>(send);
+                }
+                col0 += lb ? 2 : 0;
+            } else {
+                d[0] = c[0];
+                d[1] = c[1];
             }
-            const uint32_t row = I * 4 + (g1 ? 1 : 0) + (g2 ? 2 : 0);
+            const uint32_t row = I * 4 + (mb ? 1 : 0) + (hb ? 2 : 0);
             if (row < P.nrows_elem) {
-                const gptr<float> py = y + (size_t)row * ldy + 4 * r4 + (g0 ? 2 : 0);
+                const gptr<T> py = y + (size_t)row * ldy + col0;
                 if (yw) {
-                    *(gptr<f32x2>)py = f32x2{d[0], d[1]};
+                    typedef T T2 __attribute__((ext_vector_type(2)));
+                    *(gptr<T2>)py = T2{d[0], d[1]};
                 } else {
                     py[0] = d[0];
                     py[1] = d[1];

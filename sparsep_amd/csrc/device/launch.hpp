@@ -32,9 +32,9 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         }
         if (w.kind == WK_BCSR_VEC) {
             if constexpr (__is_same(T, float) || __is_same(T, double)) {
-                if (w.R == 4 && w.C == 4) {  // 4 x 4 blocks: the matrix-core kernel (FP32, 32 columns: vector ALUs)
-                    if constexpr (__is_same(T, float) && K == 32)
-                        hipLaunchKernelGGL((bcsr4_valu_kernel<32>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits);
+                if (w.R == 4 && w.C == 4) {  // 4 x 4 blocks: the matrix-core kernel (32 columns: vector ALUs)
+                    if constexpr (K == 32)
+                        hipLaunchKernelGGL((bcsr4_valu_kernel<T, 32>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits);
                     else
                         hipLaunchKernelGGL((bcsr_mfma_kernel<T, K>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits);
                     return hipGetLastError() == hipSuccess ? SP_OK : SP_ERR_HIP;
