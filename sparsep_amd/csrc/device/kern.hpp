@@ -651,8 +651,13 @@ __global__ __launch_bounds__(TPB) void csr_vector_kernel(const Prob *__restrict_
 // row loop of spmv/1D/CSR-row/dpu/task.c:105-202 with four entries per lane
 // and load.  (27-point stencil 64^3 x 4, FP32: 245 us with 32 lanes against
 // 301-363 us for one entry per lane and 303 us for the merge-path kernel.)
+// Q (quads): the plan found every row's length a multiple of 4 and every
+// chunk's columns c, c+1, c+2, c+3 with c a multiple of 4 (rows made of
+// aligned runs of four columns, as in a stencil with 4 unknowns per point);
+// a chunk then reads its four x values in one load instead of four gathers
+// when x is contiguous and aligned to four values.
 // ---------------------------------------------------------------------------
-template <typename T, int L>
+template <typename T, int L, bool Q = false>
 __global__ __launch_bounds__(TPB) void csr_chunk_kernel(const Prob *__restrict__ probs,
                                                         const uint32_t *__restrict__ unit_pid,
                                                         const uint32_t *__restrict__ urow, Ptrs<T> io,
@@ -669,6 +674,42 @@ __global__ __launch_bounds__(TPB) void csr_chunk_kernel(const Prob *__restrict__
     const int ldx = io.lx(P), ldy = io.ly(P);
     const uint32_t r0 = urow[u + pid], r1 = urow[u + pid + 1];
     const int lane = threadIdx.x % L, grp = threadIdx.x / L;
+    if constexpr (Q) {
+        typedef T T4 __attribute__((ext_vector_type(4)));
+        if (ldx == 1 && (uintptr_t)io.x(pid, P) % sizeof(T4) == 0) {
+            // a chunk lies inside its row or outside it; one outside reads x[0 .. 3] and adds 0
+            auto quad = [&](uint32_t k, uint32_t b, uint32_t e, A acc) {
+                uint32_t c[4];
+                T v[4];
+                load4<T>(cip, vals, k, c, v);
+                const bool ok = k >= b && k < e;
+                const T4 xv = *(gptr<const T4>)(x + (ok ? c[0] : 0u));
+#pragma unroll
+                for (int i = 0; i < 4; i++) acc = fma_acc<T>(ok ? v[i] : T(0), xv[i], acc);
+                return acc;
+            };
+            for (uint32_t r = r0 + grp; r < r1; r += G) {
+                const uint32_t b = rp[r], e = rp[r + 1];
+                A acc = A(0);
+                uint32_t k = b + 4 * (uint32_t)lane;
+                for (; k + 4 * L < e; k += 8 * L) {  // two chunks per lane in flight, both inside the row
+                    uint32_t c[2][4];
+                    T v[2][4];
+                    load4<T>(cip, vals, k, c[0], v[0]);
+                    load4<T>(cip, vals, k + 4 * L, c[1], v[1]);
+                    const T4 x0 = *(gptr<const T4>)(x + c[0][0]), x1 = *(gptr<const T4>)(x + c[1][0]);
+#pragma unroll
+                    for (int i = 0; i < 4; i++) acc = fma_acc<T>(v[0][i], x0[i], acc);
+#pragma unroll
+                    for (int i = 0; i < 4; i++) acc = fma_acc<T>(v[1][i], x1[i], acc);
+                }
+                if (k < e) acc = quad(k, b, e, acc);
+                acc = group_sum<L>(acc);
+                if (lane == 0) y[(size_t)r * ldy] = to_val<T>(acc);
+            }
+            continue;
+        }
+    }
     // chunk at k: its entries' products, those outside [b, e) zero
     auto chunk = [&](uint32_t k, uint32_t b, uint32_t e, A acc) {
         uint32_t c[4];

@@ -122,14 +122,16 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
         }
         if constexpr (K == 1) {
             if (w.ipt == 4 && w.chunk) {  // 4-entry chunks per lane (the plan's arrays: aligned and padded)
+#define SPD_CHQ(Lv) \
+    case Lv: \
+        if (w.quad) hipLaunchKernelGGL((csr_chunk_kernel<T, Lv, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); \
+        else hipLaunchKernelGGL((csr_chunk_kernel<T, Lv>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); \
+        break;
                 switch (w.lanes) {
-                case 4: hipLaunchKernelGGL((csr_chunk_kernel<T, 4>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
-                case 8: hipLaunchKernelGGL((csr_chunk_kernel<T, 8>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
-                case 16: hipLaunchKernelGGL((csr_chunk_kernel<T, 16>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
-                case 32: hipLaunchKernelGGL((csr_chunk_kernel<T, 32>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
-                case 64: hipLaunchKernelGGL((csr_chunk_kernel<T, 64>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.nunits); break;
+                    SPD_CHQ(4) SPD_CHQ(8) SPD_CHQ(16) SPD_CHQ(32) SPD_CHQ(64)
                 default: return SP_ERR_ARG;
                 }
+#undef SPD_CHQ
                 break;
             }
         }

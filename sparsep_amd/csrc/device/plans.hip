@@ -175,6 +175,20 @@ uint32_t chunk_lanes(double avg) {
     return L;
 }
 
+// Rows made of aligned runs of four columns (csr_chunk_kernel's quads): every
+// row's length a multiple of 4 and every group of four entries (which then
+// never crosses a row) the columns c, c+1, c+2, c+3 with c a multiple of 4,
+// in a matrix whose column count is a multiple of 4.
+static bool quad_rows(const sp_csr *A) {
+    if (A->nnz == 0 || A->ncols % 4) return false;
+    for (uint32_t i = 0; i < A->nrows; i++)
+        if ((A->rowptr[i + 1] - A->rowptr[i]) % 4) return false;
+    const uint32_t *c = A->colind;
+    for (uint32_t j = 0; j < A->nnz; j += 4)
+        if (c[j] % 4 || c[j + 1] != c[j] + 1 || c[j + 2] != c[j] + 2 || c[j + 3] != c[j] + 3) return false;
+    return true;
+}
+
 // Shapes the coalesced BCSR kernel takes: a block row holds whole 16-byte
 // chunks and a block at most 64 of them (see spd::Coal).  It gives a whole
 // wave to each block row, so it is chosen only when the average block row
@@ -451,6 +465,7 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
         if (ch) {
             p->ipt = p->work.ipt = 4;
             p->work.chunk = true;  // the plan's own arrays: aligned and padded (plan_alloc)
+            p->work.quad = quad_rows(A);
         }
         const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
         std::vector<uint32_t> split(P1 + 1);

@@ -38,13 +38,16 @@ struct Work {
     bool atomic = false;         // COO/BCOO: rows crossing tiles merged by atomics (no carries)
     bool chunk = false;          // CSR merge / COO: every problem's arrays aligned to 4 entries and padded
                                  // past their end (library allocations), so tiles load 4-entry chunks
+    bool quad = false;           // chunked CSR rows: every chunk is four consecutive columns from a multiple
+                                 // of 4 inside one row (x read four values at a time)
     uint32_t nprez = 0;          // atomic: rows zeroed before the main launch
     uint32_t *d_prez_pid = nullptr, *d_prez_row = nullptr;
 };
 
 // Widest right-hand-side chunk a kernel kind takes in one pass: the
 // row-granular CSR kernel keeps K accumulators per lane; FP32/FP64 4 x 4
-// BCSR blocks run on the matrix cores (8, 16 or 32 per pass); the register-blocked
+// BCSR blocks run on the matrix cores (8 or 16 per pass) or the vector ALUs
+// (32 per pass); the register-blocked
 // BCSR kernel R*K (R*K <= 64); for 8-byte values and R*C <= 16, lanes run
 // across up to 32 right-hand sides with R sums each; the kernels with carries
 // keep K partial sums per carry slot.
@@ -53,7 +56,7 @@ inline int max_rhs_chunk(const Work &w) {
     switch (w.kind) {
     case WK_CSR_VEC: return 32;
     case WK_BCSR_VEC:
-        if ((w.dtype == SP_F32 || w.dtype == SP_F64) && w.R == 4 && w.C == 4) return 32;  // matrix-core kernel
+        if ((w.dtype == SP_F32 || w.dtype == SP_F64) && w.R == 4 && w.C == 4) return 32;  // matrix-core / vector-ALU kernels
         return wide8 ? 32 : w.R <= 2 ? 32 : w.R <= 4 ? 16 : 8;
     case WK_BCOO_SEG: return 1;
     case WK_BCSR_SEG: return 1;

@@ -208,3 +208,37 @@ def test_bandwidth_probes(sp, ctx):
     out = (ctypes.c_float * 1)()
     assert sp.lib().sp_time_read(ctx.ptr, 8, 1, out) == -1  # SP_ERR_ARG
     assert sp.lib().sp_time_read(ctx.ptr, n, 0, out) == -1
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+def test_quad_rows(sp, ctx, dt):
+    """Rows made of aligned runs of four columns (a 27-point stencil with 4
+    unknowns per point; a matrix of random aligned quads with empty rows and
+    rows of 4 to 64 entries): the chunked row kernel reads x four values at a
+    time.  Every lane count, both kernel ids; x at an address that is not a
+    multiple of four values takes the gathering path.  Integers bit-exact."""
+    rng = np.random.default_rng(40 + dt)
+    nrows, ncols = 3000, 2400
+    rows, cols = [], []
+    for r in range(nrows):
+        if r % 11 == 0:
+            continue
+        q = rng.choice(ncols // 4, size=int(rng.integers(1, 17)), replace=False)
+        for c0 in np.sort(q):
+            rows += [r] * 4
+            cols += [4 * int(c0) + i for i in range(4)]
+    mats = [sp.gen_stencil27(5, 4, True, dt, seed=41), _coo(sp, nrows, ncols, rows, cols, dt, seed=42)]
+    for A in mats:
+        x = sp.gen_vector(dt, A.ncols, seed=43)
+        for kern in ("CSR-row", "CSR-nnz"):
+            for L in (4, 8, 16, 32, 64):
+                y, _ = run(sp, ctx, kern, A, x, sp.opts(algo=1, lanes_per_row=L, items_per_lane=4))
+                assert_parity(dt, y, A, x, f"{kern}/quads/L{L}/n{A.nrows}")
+        # x one value past an aligned address
+        plan = sp.Plan(ctx, "CSR-nnz", A, sp.opts(algo=1, lanes_per_row=16, items_per_lane=4))
+        xe = np.concatenate([np.zeros(1, x.dtype), x])
+        xd = sp.DeviceArray.from_numpy(ctx, xe)
+        yd = sp.DeviceArray(ctx, (A.nrows,), x.dtype)
+        plan.spmv(xd.addr + x.itemsize, yd.addr)
+        ctx.sync()
+        assert_parity(dt, yd.numpy(), A, x, f"quads/x offset/n{A.nrows}")
