@@ -891,8 +891,13 @@ __global__ __launch_bounds__(TPB) void csr_rhs_kernel(const Prob *__restrict__ p
 // row.  A thread owns IPT consecutive entries; a row is written by the
 // thread holding its last entry, with the partial sums of earlier threads
 // of the tile added in; partials of earlier tiles arrive in the fix-up.
+// QD (with CH, row-ordered tiles): the plan found every group of four
+// entries in the arrays to be one row's columns c .. c+3 with c a multiple
+// of 4; a chunk then reads its four x values in one load when x is
+// contiguous and aligned to four values.
 // ---------------------------------------------------------------------------
-template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB, bool ATOM = false, bool CH = false>
+template <typename T, int IPT, int K, bool SLICE = false, int NT = TPB, bool ATOM = false, bool CH = false,
+          bool QD = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH ? 6 : 1))) void coo_seg_kernel(const Prob *__restrict__ probs,
                                                       const uint32_t *__restrict__ unit_pid,
                                                       const uint32_t *__restrict__ cent, Ptrs<T> io,
@@ -977,13 +982,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH ? 6 : 1))
                 rv[r][3] = w[3];
                 load4<T>(cip, vals, a0 + 4 * c, cv[r], vv[r]);
             }
+            typedef T T4 __attribute__((ext_vector_type(4)));
+            if (QD && ldx == 1 && (uintptr_t)io.x(pid, P) % sizeof(T4) == 0) {
+                // every chunk of the arrays is an aligned run of four columns of one row
 #pragma unroll
-            for (int r = 0; r < R; r++)
+                for (int r = 0; r < R; r++) {
+                    const T4 q = *(gptr<const T4>)(x + cv[r][0]);
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const uint32_t t4 = 4 * (tid + r * NT) + i;  // lead + tile entry
-                    xv[r][i] = gather(t4 >= lead && t4 - lead < n ? cv[r][i] : 0u);
+                    for (int i = 0; i < 4; i++) xv[r][i] = q[i];
                 }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const uint32_t t4 = 4 * (tid + r * NT) + i;  // lead + tile entry
+                        xv[r][i] = gather(t4 >= lead && t4 - lead < n ? cv[r][i] : 0u);
+                    }
+            }
 #pragma unroll
             for (int r = 0; r < R; r++)
 #pragma unroll

@@ -166,6 +166,15 @@ static int launch_main(const Work &w, hipStream_t s, Ptrs<T> io) {
                 }
                 break;
             }
+            if (w.quad) {  // row-ordered chunk tiles reading x four values per load
+                switch (w.ipt) {
+                case 4: hipLaunchKernelGGL((coo_seg_kernel<T, 4, 1, false, TPB, false, true, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+                case 8: hipLaunchKernelGGL((coo_seg_kernel<T, 8, 1, false, TPB, false, true, true>), grid, block, 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+                case 16: hipLaunchKernelGGL((coo_seg_kernel<T, 8, 1, false, 2 * TPB, false, true, true>), grid, dim3(2 * TPB), 0, s, w.d_prob, w.d_unit_pid, w.d_c0, io, w.d_carry_row, cv, w.nunits); break;
+                default: return SP_ERR_ARG;
+                }
+                break;
+            }
             if constexpr (sizeof(T) <= 2) {  // as for CSR above
             if (w.chunk) {
                 switch (w.ipt) {

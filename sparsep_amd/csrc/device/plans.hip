@@ -179,12 +179,11 @@ uint32_t chunk_lanes(double avg) {
 // row's length a multiple of 4 and every group of four entries (which then
 // never crosses a row) the columns c, c+1, c+2, c+3 with c a multiple of 4,
 // in a matrix whose column count is a multiple of 4.
-static bool quad_rows(const sp_csr *A) {
-    if (A->nnz == 0 || A->ncols % 4) return false;
-    for (uint32_t i = 0; i < A->nrows; i++)
-        if ((A->rowptr[i + 1] - A->rowptr[i]) % 4) return false;
-    const uint32_t *c = A->colind;
-    for (uint32_t j = 0; j < A->nnz; j += 4)
+static bool quad_rows(const uint32_t *rowptr, uint32_t nrows, const uint32_t *c, uint32_t nnz, uint32_t ncols) {
+    if (nnz == 0 || ncols % 4) return false;
+    for (uint32_t i = 0; i < nrows; i++)
+        if ((rowptr[i + 1] - rowptr[i]) % 4) return false;
+    for (uint32_t j = 0; j < nnz; j += 4)
         if (c[j] % 4 || c[j + 1] != c[j] + 1 || c[j + 2] != c[j] + 2 || c[j + 3] != c[j] + 3) return false;
     return true;
 }
@@ -465,7 +464,7 @@ int build_csr_plan(sp_plan *p, const sp_csr *A) {
         if (ch) {
             p->ipt = p->work.ipt = 4;
             p->work.chunk = true;  // the plan's own arrays: aligned and padded (plan_alloc)
-            p->work.quad = quad_rows(A);
+            p->work.quad = quad_rows(A->rowptr, A->nrows, A->colind, A->nnz, A->ncols);
         }
         const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
         std::vector<uint32_t> split(P1 + 1);
@@ -541,7 +540,13 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
         if (p->opts.items_per_lane == 0 && maxlen > 4.0 * nnz / std::max(n, 1u) + 64 && nnz > (8u << 20) &&
             A->ncols <= (1u << 20) && p->algo != SP_ALGO_SEGMENTED)  // column-ordered 4096-entry tiles only
             p->opts.items_per_lane = 16;
-        const uint32_t I = pick_ipt(p->opts, 4);
+        // rows made of aligned runs of four columns: row-ordered chunk tiles reading x four
+        // values per load (unless column order is asked for, or atomics), 8 items per lane
+        // (27-point stencil 64^3 x 4: FP32 295-297 us against 363-366 us at 4 and 360 us for
+        // column-ordered tiles; FP64 401-408 against 446-456 and 437-441 us)
+        const bool quads = !atom && !p->opts.lds_slice && p->algo != SP_ALGO_SEGMENTED_SORTED &&
+                           quad_rows(rowptr.data(), n, A->colind, nnz, A->ncols);
+        const uint32_t I = pick_ipt(p->opts, quads ? 8 : 4);
         if (!I) { rc = SP_ERR_ARG; break; }
         p->ipt = p->work.ipt = I;
         const uint32_t tile = TPB * I;
@@ -550,6 +555,7 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
         uint32_t pb = 0;
         while ((1u << pb) < tile) pb++;
         bool col_order = p->algo != SP_ALGO_SEGMENTED && A->ncols <= (1ull << (32 - pb));
+        if (quads) col_order = false;
         const uint32_t P1 = p->opts.nparts ? p->opts.nparts : 1;
         // part entry ranges
         std::vector<uint32_t> pe0(P1), pe1(P1);
@@ -618,6 +624,7 @@ int build_coo_plan(sp_plan *p, const sp_coo *A0) {
             if (rowptr[i + 1] == rowptr[i]) { b.zpid.push_back(0); b.zrow.push_back(i); }
         b.add(P, units, nullptr);
         p->work.chunk = true;  // the plan's own arrays: aligned and padded (plan_alloc)
+        p->work.quad = quads;
         rc = b.finish(WK_COO_SEG, true, 1);
         if (!rc && atom) rc = atomic_prezero(p, A->rowind, units, 1, n, b.zrow);
     } while (0);

@@ -38,8 +38,9 @@ struct Work {
     bool atomic = false;         // COO/BCOO: rows crossing tiles merged by atomics (no carries)
     bool chunk = false;          // CSR merge / COO: every problem's arrays aligned to 4 entries and padded
                                  // past their end (library allocations), so tiles load 4-entry chunks
-    bool quad = false;           // chunked CSR rows: every chunk is four consecutive columns from a multiple
-                                 // of 4 inside one row (x read four values at a time)
+    bool quad = false;           // chunked CSR rows / row-ordered COO chunk tiles: every chunk is four
+                                 // consecutive columns from a multiple of 4 inside one row (x read four
+                                 // values at a time)
     uint32_t nprez = 0;          // atomic: rows zeroed before the main launch
     uint32_t *d_prez_pid = nullptr, *d_prez_row = nullptr;
 };

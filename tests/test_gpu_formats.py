@@ -272,3 +272,43 @@ def test_bcsr4_spmm_matrix_cores(sp, ctx, dt, k):
             kw = dict(vert=3) if kern[0] in "DRB" and kern[1] != "C" else {}
             Y, _ = run(sp, ctx, kern, A, X, sp.opts(row_block_size=4, col_block_size=4, **kw), k=k)
             assert_parity_multi(dt, Y, A, X, f"{kern} k{k} n{A.nrows}")
+
+
+@pytest.mark.parametrize("dt", [0, 1, 2, 3, 4, 5])
+def test_coo_quad_rows(sp, ctx, dt):
+    """COO of rows made of aligned runs of four columns (a 27-point stencil
+    with 4 unknowns per point; random aligned quads with empty rows): the
+    automatic plan takes row-ordered chunk tiles reading x four values per
+    load, for every tile size and partition knob; integers bit-exact with the
+    oracle and equal to the column-ordered plan; x at an address that is not a
+    multiple of four values takes the gathering path."""
+    rng = np.random.default_rng(60 + dt)
+    nrows, ncols = 2500, 2000
+    rows, cols = [], []
+    for r in range(nrows):
+        if r % 7 == 0:
+            continue
+        q = rng.choice(ncols // 4, size=int(rng.integers(1, 12)), replace=False)
+        for c0 in np.sort(q):
+            rows += [r] * 4
+            cols += [4 * int(c0) + i for i in range(4)]
+    vals = rng.integers(-4, 5, size=len(rows)).astype(sp.NP_DTYPE[dt])
+    Q = sp.Coo.from_arrays(nrows, ncols, np.asarray(rows), np.asarray(cols), vals, sort=True)
+    for A in (sp.gen_stencil27(5, 4, True, dt, seed=61), Q):
+        x = sp.gen_vector(dt, A.ncols, seed=62)
+        for kern in ("COO-row", "COO-nnz-rgrn", "COO-nnz"):
+            for kw in (dict(), dict(nparts=16), dict(nparts=64, nsub=3)):
+                for ipt in (4, 8, 16):
+                    y, plan = run(sp, ctx, kern, A, x, sp.opts(items_per_lane=ipt, **kw))
+                    assert plan.info()["algo"] == 2, (kern, kw, ipt)
+                    assert_parity(dt, y, A, x, f"{kern}/quads/{kw}/ipt{ipt}")
+                    if dt <= 3:
+                        y3, _ = run(sp, ctx, kern, A, x, sp.opts(algo=3, items_per_lane=ipt, **kw))
+                        assert np.array_equal(y, y3), (kern, kw, ipt)
+        plan = sp.Plan(ctx, "COO-nnz", A, sp.opts())
+        xe = np.concatenate([np.zeros(1, x.dtype), x])
+        xd = sp.DeviceArray.from_numpy(ctx, xe)
+        yd = sp.DeviceArray(ctx, (A.nrows,), x.dtype)
+        plan.spmv(xd.addr + x.itemsize, yd.addr)
+        ctx.sync()
+        assert_parity(dt, yd.numpy(), A, x, f"COO quads/x offset/n{A.nrows}")
