@@ -920,9 +920,24 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
     const uint32_t nt = T->ntiles, th = T->tile_height;
     Builder b(p);
     void *d;
-    // 2D CSR tiles in the chunked row kernel (VECTOR with items_per_lane 4, asked for: AUTO
-    // keeps the merge path, which measured the same on the 27-point stencil at two stripes,
-    // 313 us each), with every tile's entries starting at a multiple of 4 in the arrays
+    // every tile's rows made of aligned runs of four columns, in stripes starting at a multiple
+    // of 4: the chunked row kernel reads x four values per load
+    bool quads = tk.format == SP_FMT_CSR && !p->opts.lds_slice &&
+                 (p->algo == SP_ALGO_AUTO || (p->algo == SP_ALGO_VECTOR && p->opts.items_per_lane == 4));
+    for (uint32_t t = 0; t < nt && quads; t++) {
+        const uint32_t c = t % vert, w0 = T->vert_tile_widths[c], w = T->vert_tile_widths[c + 1] - w0;
+        const uint32_t cnt = T->nnzs_per_partition[t];
+        quads = w0 % 4 == 0 &&
+                (cnt == 0 || quad_rows(T->drowptr + (size_t)t * (th + 1), th, T->dcolind + T->nnz_offset[t], cnt, w));
+    }
+    if (quads && p->algo == SP_ALGO_AUTO) {
+        p->algo = SP_ALGO_VECTOR;
+        p->opts.items_per_lane = 4;
+    }
+    // 2D CSR tiles in the chunked row kernel (VECTOR with items_per_lane 4: asked for, or AUTO
+    // with quads; otherwise AUTO keeps the merge path, which measured the same on the 27-point
+    // stencil at two stripes, 313 us each, with four gathers per chunk), with every tile's
+    // entries starting at a multiple of 4 in the arrays
     const bool csr_chunk = tk.format == SP_FMT_CSR && p->algo == SP_ALGO_VECTOR && p->opts.items_per_lane == 4 &&
                            !p->opts.lds_slice;
     std::vector<uint64_t> e0c;  // chunked layout: each tile's first entry in the uploaded arrays
@@ -1032,6 +1047,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             L = csr_chunk ? chunk_lanes(n ? (double)A0->nnz / n / vert : 1.0) : auto_lanes(n ? (double)A0->nnz / n / vert : 1.0);
         if (csr_chunk && (L < 4 || L > 64 || (L & (L - 1)))) { rc = SP_ERR_ARG; break; }
         p->work.chunk = csr_chunk;
+        p->work.quad = csr_chunk && quads;
         bool coal = false;
         if (kind == WK_BCSR_VEC && L == 0) {
             const double avg = T->num_block_rows ? (double)T->num_blocks / ((double)T->num_block_rows * nt) : 1.0;
