@@ -940,6 +940,20 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
     // entries starting at a multiple of 4 in the arrays
     const bool csr_chunk = tk.format == SP_FMT_CSR && p->algo == SP_ALGO_VECTOR && p->opts.items_per_lane == 4 &&
                            !p->opts.lds_slice;
+    // 2D COO tiles of aligned four-column runs (every group of four entries of a tile one row's
+    // columns c .. c+3, c a multiple of 4, stripes starting at a multiple of 4): row-ordered
+    // 4-entry chunk tiles reading x four values per load, in the same padded layout
+    bool coo_quads = tk.format == SP_FMT_COO && !p->opts.lds_slice && p->algo != SP_ALGO_SEGMENTED_ATOMIC &&
+                     p->algo != SP_ALGO_SEGMENTED_SORTED;
+    for (uint32_t t = 0; t < nt && coo_quads; t++) {
+        const uint32_t w0 = T->vert_tile_widths[t % vert], cnt = T->nnzs_per_partition[t];
+        const uint32_t *r = T->drowind + T->nnz_offset[t], *c = T->dcolind + T->nnz_offset[t];
+        coo_quads = w0 % 4 == 0 && cnt % 4 == 0;
+        for (uint32_t j = 0; j < cnt && coo_quads; j += 4)
+            coo_quads = r[j + 1] == r[j] && r[j + 2] == r[j] && r[j + 3] == r[j] && c[j] % 4 == 0 &&
+                        c[j + 1] == c[j] + 1 && c[j + 2] == c[j] + 2 && c[j + 3] == c[j] + 3;
+    }
+    const bool padded = csr_chunk || coo_quads;
     std::vector<uint64_t> e0c;  // chunked layout: each tile's first entry in the uploaded arrays
     do {
         // upload the tiling's arrays once; tiles point into them
@@ -955,11 +969,11 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             if ((rc = plan_upload(p, T->browptr, (size_t)nt * (T->num_block_rows + 1) * 4, &d))) break;
             d_ptr = (const uint32_t *)d;
         }
-        if (tk.format == SP_FMT_COO || tk.format == SP_FMT_BCOO) {
+        if ((tk.format == SP_FMT_COO && !coo_quads) || tk.format == SP_FMT_BCOO) {
             if ((rc = plan_upload(p, tk.format == SP_FMT_COO ? T->drowind : T->browind, nent * 4, &d))) break;
             d_rind = (const uint32_t *)d;
         }
-        if (csr_chunk) {
+        if (padded) {
             // tile t's entries at e0c[t], a multiple of 4 (the gaps hold zeros)
             e0c.resize(nt);
             uint64_t tot = 0;
@@ -973,6 +987,13 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
                 const uint64_t src = T->nnz_offset[t], cnt = T->nnzs_per_partition[t];
                 memcpy(ci.data() + e0c[t], T->dcolind + src, cnt * 4);
                 memcpy(va.data() + e0c[t] * sv, (const char *)T->dval + src * sv, cnt * sv);
+            }
+            if (coo_quads) {
+                std::vector<uint32_t> ri(tot, 0);
+                for (uint32_t t = 0; t < nt; t++)
+                    memcpy(ri.data() + e0c[t], T->drowind + T->nnz_offset[t], (size_t)T->nnzs_per_partition[t] * 4);
+                if ((rc = plan_upload(p, ri.data(), tot * 4, &d))) break;
+                d_rind = (const uint32_t *)d;
             }
             if ((rc = plan_upload(p, ci.data(), tot * 4, &d))) break;
             d_cind = (const uint32_t *)d;
@@ -1006,7 +1027,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             col_order = p->algo == SP_ALGO_SEGMENTED_SORTED;
         } else if (tk.format == SP_FMT_COO) {
             kind = WK_COO_SEG;
-            col_order = p->algo != SP_ALGO_SEGMENTED;  // as for 1D COO
+            col_order = p->algo != SP_ALGO_SEGMENTED && !coo_quads;  // as for 1D COO
             p->algo = atom ? SP_ALGO_SEGMENTED_ATOMIC : SP_ALGO_SEGMENTED;
         } else if (tk.format == SP_FMT_BCSR) {
             kind = WK_BCSR_VEC;
@@ -1046,8 +1067,8 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
         if (kind == WK_CSR_VEC && L == 0)
             L = csr_chunk ? chunk_lanes(n ? (double)A0->nnz / n / vert : 1.0) : auto_lanes(n ? (double)A0->nnz / n / vert : 1.0);
         if (csr_chunk && (L < 4 || L > 64 || (L & (L - 1)))) { rc = SP_ERR_ARG; break; }
-        p->work.chunk = csr_chunk;
-        p->work.quad = csr_chunk && quads;
+        p->work.chunk = padded;
+        p->work.quad = (csr_chunk && quads) || coo_quads;
         bool coal = false;
         if (kind == WK_BCSR_VEC && L == 0) {
             const double avg = T->num_block_rows ? (double)T->num_blocks / ((double)T->num_block_rows * nt) : 1.0;
@@ -1071,7 +1092,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
             P.yoff = (int64_t)c * n + row_off;
             std::vector<uint32_t> k0, k1;
             if (!blk) {
-                const uint64_t e0 = T->nnz_offset[t], e0u = csr_chunk ? e0c[t] : e0;
+                const uint64_t e0 = T->nnz_offset[t], e0u = padded ? e0c[t] : e0;
                 P.nent = T->nnzs_per_partition[t];
                 P.cind = d_cind + e0u;
                 P.val = d_val + e0u * sv;
@@ -1098,7 +1119,7 @@ int build_tiled_plan(sp_plan *p, const sp_coo *A0) {
                         push_cut(k0, rows);
                     }
                 } else {  // COO tile
-                    P.rind = d_rind + e0;
+                    P.rind = d_rind + e0u;
                     const uint32_t cnt = P.nent;
                     if (cnt) {
                         if (tk.tiling == SP_TILE_D) {

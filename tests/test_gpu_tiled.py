@@ -56,11 +56,12 @@ def test_2d_csr_algorithms_and_blocks(sp, ctx, dt):
 
 @pytest.mark.parametrize("k", [2, 5])
 def test_2d_spmm(sp, ctx, k):
-    A = sp.gen_rmat(10, 9000, dtype=4, seed=4)
-    X = np.stack([sp.gen_vector(4, A.ncols, seed=60 + q) for q in range(k)], axis=1)
-    for kern in K2D:
-        Y, _ = run(sp, ctx, kern, A, X, sp.opts(vert=3), k=k)
-        assert_parity_multi(4, Y, A, X, kern)
+    # the stencil's tiles are made of aligned four-column runs (padded chunk layouts)
+    for A in (sp.gen_rmat(10, 9000, dtype=4, seed=4), sp.gen_stencil27(6, 4, True, 4, seed=5)):
+        X = np.stack([sp.gen_vector(4, A.ncols, seed=60 + q) for q in range(k)], axis=1)
+        for kern in K2D:
+            Y, _ = run(sp, ctx, kern, A, X, sp.opts(vert=3), k=k)
+            assert_parity_multi(4, Y, A, X, f"{kern}/n{A.nrows}")
 
 
 def test_2d_edge_cases(sp, ctx):
