@@ -1588,7 +1588,7 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
 // and an empty slot of the last step skips its products; otherwise empty
 // slots read a block of zeros and x rows past the matrix read as zeros.
 // The FP32 kernel issues its vector ALU instructions most of the time
-// (counters on the 27-point stencil 64^3 x 4: 179M instructions, 86% of the
+// (counters on the 27-point stencil 64^3 x 4: 179M instructions, 87% of the
 // SIMD cycles, before the two changes above): they took it from 409-417 us to
 // 345 us.  The FP32 matrix-core kernel takes 511-515 us: its 4-byte x loads (one column per
 // lane, as the 4 x 4 x 1 instruction wants its B operand) issue four times as
