@@ -1380,12 +1380,13 @@ __global__ __launch_bounds__(TPB) void rows_rhs_kernel(const Prob *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// BCSR 4 x 4 blocks times K >= 8 right-hand sides on the matrix cores: the
+// BCSR 4 x 4 blocks times 8 or 16 right-hand sides on the matrix cores: the
 // block-row product of the reference (spmv/1D/BCSR-nnz/dpu/task.c:337-370)
 // as a dense 4 x 4 x K contraction per block.  A wave takes a block row; its
 // block columns are read 64 at a time, one per lane, and handed out by
-// shuffles.  The matrix arrays are read once per pass for up to 32
-// right-hand sides.
+// shuffles.  (32 right-hand sides go to bcsr4_valu_kernel, which is faster on
+// the 27-point stencil: FP32 331-345 against 514 us, FP64 633-637 against
+// 727 us.)
 //
 // FP32, v_mfma_f32_4x4x1_16b_f32: 16 independent 4x4x1 products per
 // instruction; product m = lane / 4 gets A[i] from lane 4m + i and B[j] from
@@ -1406,10 +1407,8 @@ __global__ __launch_bounds__(TPB) void rows_rhs_kernel(const Prob *__restrict__ 
 // l = 16h + 4m + o supplies block[o][h] (A, from the block values staged in
 // LDS 32 blocks at a time in 16-byte chunks) and x row (4 J + h) (B); the
 // columns that B's j = o stands for are a relabelling of the right-hand
-// sides: with 32 of them, instruction n of a block takes columns
-// 8m + 2o + n, so a lane reads its two B values (and writes its two Y
-// values) as one 16-byte piece.  16: one instruction, columns 4m + o.
-// 8: two blocks per instruction (products 0-1 and 2-3), columns 4(m%2) + o.
+// sides: 16: one instruction per block, columns 4m + o; 8: two blocks per
+// instruction (products 0-1 and 2-3), columns 4(m%2) + o.
 // ---------------------------------------------------------------------------
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
@@ -1420,7 +1419,7 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
                                                         const uint32_t *__restrict__ unit_pid,
                                                         const uint32_t *__restrict__ urow, Ptrs<T> io,
                                                         uint32_t nunits) {
-    static_assert(K == 8 || K == 16 || K == 32, "8, 16 or 32 right-hand sides per pass");
+    static_assert(K == 8 || K == 16, "8 or 16 right-hand sides per pass");
     static_assert(sizeof(T) == 4 || sizeof(T) == 8, "FP32 or FP64");
     constexpr int NW = TPB / WAVE;
     constexpr int NB64 = 32;  // FP64: blocks per LDS batch (4 KB of values per wave)
@@ -1502,19 +1501,13 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
             }
         } else {
             constexpr int BPS = K == 8 ? 2 : 1;  // matrix blocks per instruction
-            constexpr int NI = K == 32 ? 2 : 1;  // instructions per block
             constexpr int U = 4 / BPS;           // steps whose loads go out together
             double *const sv = s_val[wid];
-            // whole 16-byte pieces of x and y rows (uniform over the launch)
-            const bool xw = ((uintptr_t)io.x(pid, P) % 16 == 0) && (ldx % 2 == 0);
-            const bool yw = ((uintptr_t)io.y(pid, P) % 16 == 0) && (ldy % 2 == 0);
             const int h = lane / 16, m = (lane / 4) % 4, o = lane % 4;
             const int gblk = BPS == 2 ? m / 2 : 0;  // block of this product within the step
-            // right-hand side column of instruction n in this lane (a relabelling, see above)
-            auto colq = [&](int n) { return K == 32 ? 8 * m + 2 * o + n : K == 16 ? 4 * m + o : 4 * (m % 2) + o; };
-            double acc[NI];
-#pragma unroll
-            for (int n = 0; n < NI; n++) acc[n] = 0.0;
+            // right-hand side column of this lane (a relabelling, see above)
+            const int colq = K == 16 ? 4 * m + o : 4 * (m % 2) + o;
+            double acc = 0.0;
             for (uint32_t base = b; base < e; base += NB64) {
                 const uint32_t nb = min((uint32_t)NB64, e - base);
                 // the batch's values to LDS in 16-byte chunks, its block columns to lanes
@@ -1525,7 +1518,7 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 for (uint32_t t0 = 0; t0 < nb; t0 += BPS * U) {
-                    double xv[U][NI];
+                    double xv[U];
                     bool act[U];
                     uint32_t sidx[U];
 #pragma unroll
@@ -1535,36 +1528,19 @@ __global__ __launch_bounds__(TPB) void bcsr_mfma_kernel(const Prob *__restrict__
                         sidx[w] = act[w] ? sl : 0u;
                         const uint32_t row = (uint32_t)__shfl((int)Jl, (int)sidx[w], WAVE) * 4 + h;
                         const bool ok = act[w] && row < ncols;
-                        if (NI == 2 && xw) {
-                            f64x2 v = {0.0, 0.0};
-                            if (ok) v = *((gptr<const f64x2>)(x + (size_t)row * ldx + colq(0)));
-                            xv[w][0] = v[0];
-                            xv[w][NI - 1] = v[1];
-                        } else {
-#pragma unroll
-                            for (int n = 0; n < NI; n++) xv[w][n] = ok ? (double)x[(size_t)row * ldx + colq(n)] : 0.0;
-                        }
+                        xv[w] = ok ? (double)x[(size_t)row * ldx + colq] : 0.0;
                     }
 #pragma unroll
                     for (int w = 0; w < U; w++) {
                         const double av = act[w] ? sv[sidx[w] * 16 + o * 4 + h] : 0.0;
-#pragma unroll
-                        for (int n = 0; n < NI; n++) acc[n] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, xv[w][n], acc[n], 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f64_4x4x4f64(av, xv[w], acc, 0, 0, 0);
                     }
                 }
                 __builtin_amdgcn_wave_barrier();  // the next batch overwrites the staged values
             }
             const uint32_t row = I * 4 + h;
-            if constexpr (BPS == 2) acc[0] += lane_xor<8>(acc[0]);  // the other block of the step: m ^ 2
-            if ((BPS == 1 || gblk == 0) && row < P.nrows_elem) {
-                if (NI == 2 && yw) {
-                    const f64x2 v = {acc[0], acc[NI - 1]};
-                    *((gptr<f64x2>)(y + (size_t)row * ldy + colq(0))) = v;
-                } else {
-#pragma unroll
-                    for (int n = 0; n < NI; n++) y[(size_t)row * ldy + colq(n)] = (T)acc[n];
-                }
-            }
+            if constexpr (BPS == 2) acc += lane_xor<8>(acc);  // the other block of the step: m ^ 2
+            if ((BPS == 1 || gblk == 0) && row < P.nrows_elem) y[(size_t)row * ldy + colq] = (T)acc;
         }
     }
     }
